@@ -1,0 +1,226 @@
+"""Benchmark of the batched radix top-k on one MI355X.
+
+Default workload: batched uneven rows (2048 rows, len = floor(2^U(10,24)),
+k = floor(2^U(0, log2 min(len, 4096))), N(0,1) fp32, largest, sorted),
+one rk_topk_batched call per step with inputs resident in HBM.
+
+Prints one JSON line with:
+  value        whole-job keys/s (sum of row lengths / time per step)
+  roofline     the dominant kernel (the streaming filter): algorithmic bytes
+               per launch / its average duration, measured with HIP events
+               recorded around it inside every timed step
+  cpu_baseline the multithreaded CPU oracle (std::nth_element + std::sort)
+               on a bounded sample of the same rows, on this host's cores
+  plain        the same rows with one RadiK multi-pass call per row
+Progress lines go to stderr.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "top-k input keys/s + % HBM roofline (batched vs plain); serving p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    sys.stderr.write(f"[bench {time.strftime('%H:%M:%S')}] {msg}\n")
+    sys.stderr.flush()
+
+
+def make_workload(name: str):
+    import radik_amd.workloads as W
+    table = {
+        "c4_f32": lambda: W.c4("f32"),
+        "c4_bf16": lambda: W.c4("bf16"),
+        "c2": lambda: W.c2(),
+        "c1": lambda: W.c1(),
+        "c3": lambda: W.c3(),
+    }
+    return table[name]()
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c4_f32", choices=["c4_f32", "c4_bf16", "c2", "c1", "c3"])
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample-keys", type=float, default=1.0e9,
+                    help="bound on the keys in the CPU baseline's sample")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; rows are independent and "
+                          "the multi-GPU design (row sharding) is described in DESIGN.md",
+                          "n_gpus": args.gpus}))
+        return 2
+
+    import numpy as np
+    import torch
+
+    import radik_amd as rk
+    import radik_amd.workloads as W
+
+    torch.cuda.set_device(0)
+    w = make_workload(args.workload)
+    es = 4 if w.dtype == "f32" else 2
+    dt = {"f32": rk.RK_F32, "bf16": rk.RK_BF16, "f16": rk.RK_F16}[w.dtype]
+    log(f"workload {w.name}: {w.rows} rows, {w.total} keys, max k {w.max_k}")
+    vals = W.materialize(w)
+    off = w.offsets()
+    oo = w.out_offsets()
+    d_off = torch.from_numpy(off).cuda()
+    d_k = torch.from_numpy(w.ks).cuda()
+    d_oo = torch.from_numpy(oo).cuda()
+    n_out = int(oo[-1])
+    out_v = torch.empty(n_out, dtype=vals.dtype, device="cuda")
+    out_i = torch.empty(n_out, dtype=torch.int32, device="cuda")
+    flags = rk.RK_LARGEST | rk.RK_SORTED
+    ws_bytes = rk.workspace_size(dt, w.rows, w.total, w.max_k, flags)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+    L = rk.lib()
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    log(f"inputs resident: {vals.numel() * es / 1e9:.2f} GB values, workspace {ws_bytes / 1e6:.1f} MB")
+
+    def step():
+        rc = L.rk_topk_batched(dt, vals.data_ptr(), d_off.data_ptr(), d_k.data_ptr(), w.rows,
+                               w.total, w.max_k, flags, None, out_v.data_ptr(), out_i.data_ptr(),
+                               d_oo.data_ptr(), ws.data_ptr(), ws_bytes, sp)
+        if rc != 0:
+            raise rk.RadikError(rc, "rk_topk_batched")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    st = rk.status(ws)
+    if st[0] != 0:
+        raise RuntimeError(f"device status {st}")
+    log(f"warmup done ({args.warmup} steps); fallback rows {st[1]}, ordered rows {st[2]}")
+
+    rk.timing_read()  # drop anything recorded before
+    rk.timing_enable(True)
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    for _ in range(args.steps):
+        step()
+    t1.record(stream)
+    torch.cuda.synchronize()
+    rk.timing_enable(False)
+    elapsed_ms = t0.elapsed_time(t1)
+    filt_ms, filt_n = rk.timing_read()
+    ms_step = elapsed_ms / args.steps
+    keys_s = w.total / (ms_step / 1e3)
+    log(f"batched: {ms_step:.3f} ms/step, {keys_s / 1e9:.2f} Gkeys/s")
+
+    # Dominant kernel: the streaming filter.  Per launch it reads every key of
+    # the rows longer than the LDS-direct limit once: sizeof(T) bytes per key.
+    small_max = rk.max_k()
+    long_keys = int(w.lens[w.lens > small_max].sum())
+    filt_bytes = long_keys * es
+    filt_avg_ms = filt_ms / max(filt_n, 1)
+    achieved = filt_bytes / (filt_avg_ms / 1e3) / 1e9
+    alg_bytes = w.algorithmic_bytes()
+    pipeline_gbs = alg_bytes / (ms_step / 1e3) / 1e9
+
+    result = {
+        "metric": METRIC,
+        "value": keys_s,
+        "unit": "keys/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": w.dtype,
+        "data": "synthetic: seeded N(0,1) generated on the device (counter-based SplitMix64)",
+        "config": {"workload": w.name, "rows": w.rows, "total_keys": w.total,
+                   "len": "floor(2^U(10,24))", "k": "floor(2^U(0, log2 min(len,4096)))",
+                   "largest": True, "sorted": True, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "k_filter", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes_per_launch": filt_bytes,
+                     "avg_launch_ms": filt_avg_ms, "launches_timed": filt_n},
+        "pipeline": {"algorithmic_bytes": alg_bytes, "achieved_gbs": pipeline_gbs,
+                     "roofline_frac": pipeline_gbs / HBM_PEAK_GBS,
+                     "fallback_rows": st[1], "ordered_rows": st[2]},
+    }
+
+    if not args.no_plain and args.plain_steps > 0:
+        # One exact multi-pass call per row (RadiK's per-query use of
+        # topKRadixSelectL with taskNum = 1, radik/test/test_radik.cu:164-183).
+        pflags = flags | rk.RK_EXACT
+        p_ws = rk.workspace_size(dt, 1, int(w.lens.max()), w.max_k, pflags)
+        pws = torch.empty(p_ws, dtype=torch.uint8, device="cuda")
+        es_v = vals.element_size()
+
+        def plain_step():
+            for r in range(w.rows):
+                rc = L.rk_topk_batched(dt, vals.data_ptr(), d_off.data_ptr() + 8 * r,
+                                       d_k.data_ptr() + 4 * r, 1, int(w.lens[r]), int(w.ks[r]),
+                                       pflags, None, out_v.data_ptr(), out_i.data_ptr(),
+                                       d_oo.data_ptr() + 8 * r, pws.data_ptr(), p_ws, sp)
+                if rc != 0:
+                    raise rk.RadikError(rc, "plain rk_topk_batched")
+        del es_v
+        plain_step()
+        torch.cuda.synchronize()
+        p0 = torch.cuda.Event(enable_timing=True)
+        p1 = torch.cuda.Event(enable_timing=True)
+        p0.record(stream)
+        for _ in range(args.plain_steps):
+            plain_step()
+        p1.record(stream)
+        torch.cuda.synchronize()
+        p_ms = p0.elapsed_time(p1) / args.plain_steps
+        result["plain"] = {"ms_per_step": p_ms, "keys_per_s": w.total / (p_ms / 1e3),
+                           "batched_speedup": p_ms / ms_step,
+                           "what": "one RK_EXACT (multi-pass) call per row, same stream"}
+        log(f"plain per-row: {p_ms:.2f} ms/step ({p_ms / ms_step:.1f}x slower)")
+
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        # bounded sample: every s-th row, at most cpu_sample_keys keys
+        stride = max(1, int(np.ceil(w.total / args.cpu_sample_keys)))
+        rows = np.arange(0, w.rows, stride)
+        lens = w.lens[rows]
+        s_off = np.zeros(len(rows) + 1, np.int64)
+        s_off[1:] = np.cumsum(lens)
+        host = np.empty(int(s_off[-1]), dtype=np.uint32 if es == 4 else np.uint16)
+        iview = vals.view(torch.int32 if es == 4 else torch.int16)
+        for j, r in enumerate(rows):
+            host[s_off[j]:s_off[j + 1]] = iview[off[r]:off[r + 1]].cpu().numpy().view(host.dtype)
+        nth = min(16, orc.hardware_threads())
+        c0 = time.perf_counter()
+        orc.topk(host, s_off, w.ks[rows], dt=dt, flags=orc.LARGEST | orc.SORTED, nthreads=nth)
+        c_s = time.perf_counter() - c0
+        result["cpu_baseline"] = {
+            "value": float(s_off[-1]) / c_s, "unit": "keys/s", "cores": nth, "kind": "port",
+            "sample": f"every {stride}th row of the workload ({len(rows)} rows, "
+                      f"{int(s_off[-1])} keys), std::nth_element + std::sort per row, "
+                      f"{nth} threads, {c_s:.2f} s"}
+        log(f"cpu baseline: {float(s_off[-1]) / c_s / 1e9:.3f} Gkeys/s on {nth} threads")
+
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,145 @@
+/*
+ * radik_amd — C ABI of the MI355X (gfx950) radix top-k.
+ *
+ * Every entry point takes plain pointers and sizes; device pointers are HIP
+ * device memory; `stream` is a hipStream_t (NULL = the null stream).  No
+ * call allocates device memory, synchronises the host, or throws: errors are
+ * returned as RK_E_* codes.  All row metadata of the batched entry points is
+ * read on the device, so a call can be captured into a hipGraph and replayed.
+ *
+ * Semantics (identical for every entry point):
+ *   - a row's selection key is the order-preserving flip of its bit pattern
+ *     (radik/RadixSelect/utils.cuh:329-334); NaN ranks below every number in
+ *     both directions;
+ *   - the min(len, k) best elements are selected, ties at the k-th key broken
+ *     by the lower row position (the reference's filter admits ties by an
+ *     atomic race, radik/RadixSelect/radixselect_l.cuh:417-421);
+ *   - RK_SORTED: ordered by key in the RK_ASCEND / descending direction,
+ *     equal keys by position ascending; otherwise in position order;
+ *   - indices are row-local positions, or idx_in[row_start + position];
+ *   - output slots beyond min(len, k) are filled with value 0 / index -1
+ *     (radik/RadixSelect/topk_radixselect.h:312-320).
+ */
+#ifndef RADIK_AMD_H_
+#define RADIK_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* element types */
+enum { RK_F32 = 0, RK_F16 = 1, RK_BF16 = 2 };
+
+/* flags: RK_LARGEST mirrors the LARGEST template flag, RK_SORTED the
+ * needSorting argument and RK_ASCEND the ASCEND template flag of
+ * topKRadixSelectL (radik/RadixSelect/topk_radixselect.h:40-70).
+ * RK_EXACT forces RadiK's multi-pass histogram path (no sampled threshold):
+ * the "plain" kernel the batched path is measured against. */
+enum { RK_LARGEST = 1u, RK_SORTED = 2u, RK_ASCEND = 4u, RK_EXACT = 8u };
+
+/* return codes */
+enum {
+  RK_OK = 0,
+  RK_E_ARG = -1,         /* bad argument (null pointer, negative size, ...) */
+  RK_E_WORKSPACE = -2,   /* workspace missing or smaller than required */
+  RK_E_UNSUPPORTED = -3, /* k above rk_max_k(), or dtype unknown */
+  RK_E_ALIGN = -4,       /* value pointer not aligned to its element size */
+  RK_E_HIP = -5,         /* a HIP runtime call failed */
+  RK_E_DEVICE = -6       /* device-side validation failed (rk_topk_status) */
+};
+
+/* Largest k any entry point accepts on the current device (16384 on MI355X). */
+int32_t rk_max_k(void);
+
+/* Workspace bytes for a batched call over at most `num_rows` rows with at
+ * most `total_len` elements in total and every k <= max_k.
+ * Replaces getRadixSelectLWorkSpaceSize (topk_radixselect.h:22-33), which
+ * sizes by num_rows * longest_row; this sizes by total length and bounded
+ * candidate slots, so uneven batches need no num_rows * max_len buffer. */
+int rk_topk_workspace_size(int dtype, int32_t num_rows, int64_t total_len,
+                           int32_t max_k, uint32_t flags, size_t* bytes);
+
+/* Batched top-k over uneven rows; all metadata on the device.
+ * Replaces topKRadixSelectL (topk_radixselect.h:59-323) for batches:
+ *   vals      device, concatenated rows (valIn)
+ *   row_off   device int64[num_rows + 1]: row r is vals[row_off[r], row_off[r+1])
+ *             (the reference builds int offsets from a host taskLen[], :90-98)
+ *   row_k     device int32[num_rows]: per-row k (the reference has one K, :106-112)
+ *   total_len bound on row_off[num_rows] - row_off[0]; max_k bound on every k
+ *   idx_in    optional device int32, parallel to vals (idxIn / WITHIDXIN)
+ *   out_vals, out_idx  device; row r writes [out_off[r], out_off[r+1])
+ *   out_off   device int64[num_rows + 1]; each row needs >= min(len, k) slots
+ *   workspace device, >= rk_topk_workspace_size(...) bytes
+ * Stream ordered, asynchronous, graph-capturable. */
+int rk_topk_batched(int dtype, const void* vals, const int64_t* row_off,
+                    const int32_t* row_k, int32_t num_rows, int64_t total_len,
+                    int32_t max_k, uint32_t flags, const int32_t* idx_in,
+                    void* out_vals, int32_t* out_idx, const int64_t* out_off,
+                    void* workspace, size_t ws_bytes, void* stream);
+
+/* Same, with each row at its own device address (row_ptrs, row_len and row_k
+ * are device arrays): the form a serving loop uses for resident requests.
+ * No counterpart in the reference (it takes one contiguous valIn). */
+int rk_topk_rows(int dtype, const void* const* row_ptrs, const int64_t* row_len,
+                 const int32_t* row_k, int32_t num_rows, int64_t total_len,
+                 int32_t max_k, uint32_t flags, void* out_vals,
+                 int32_t* out_idx, const int64_t* out_off, void* workspace,
+                 size_t ws_bytes, void* stream);
+
+/* Workspace bytes for rk_topk_compat: same arguments as
+ * getRadixSelectLWorkSpaceSize<ValType>(K, n, taskNum, &bytes)
+ * (topk_radixselect.h:22-33), n = the longest row. */
+int rk_compat_workspace_size(int dtype, int32_t K, int32_t n, int32_t taskNum,
+                             uint32_t flags, size_t* bytes);
+
+/* The reference's surface: host taskLen[], one K, output [taskNum][K] with
+ * the tail fill; mirrors topKRadixSelectL(valIn, idxIn, valOut, idxOut,
+ * workSpace, taskLen, taskNum, K, stream, needSorting)
+ * (topk_radixselect.h:59-70).  Copies taskNum+1 offsets to the device from
+ * host memory, so unlike rk_topk_batched it is not graph-capturable. */
+int rk_topk_compat(int dtype, const void* valIn, const int32_t* idxIn,
+                   void* valOut, int32_t* idxOut, void* workspace,
+                   size_t ws_bytes, const int32_t* taskLen, int32_t taskNum,
+                   int32_t K, uint32_t flags, void* stream);
+
+/* Waits for `stream`, then reads the device status of the last call that
+ * used `workspace`: 0 when every row was valid.  Optional outputs: rows that
+ * left the sampled path for exact passes, and rows that needed the ordered
+ * emit.  No counterpart in the reference (it throws, error_check.hpp:14-58). */
+int rk_topk_status(const void* workspace, void* stream, uint32_t* status,
+                   uint32_t* fallback_rows, uint32_t* ordered_rows);
+
+/* Human-readable text of a return code. */
+const char* rk_strerror(int code);
+
+/* Tunables of the batched path (tile size, persistent grid, sample shape,
+ * digit width, loads in flight).  Values <= 0 keep the default. */
+typedef struct rk_tune {
+  int32_t tile;
+  int32_t grid;
+  int32_t sample_groups;
+  int32_t group_elems;
+  int32_t hist_bits;
+  int32_t unroll;
+} rk_tune;
+
+/* Reads / overrides the tunables used by subsequent calls in this process
+ * (the autotuner sets them per shape bucket). */
+int rk_get_tune(rk_tune* out);
+int rk_set_tune(const rk_tune* in);
+
+/* Profiling hook (no counterpart in the reference): while enabled, every
+ * batched call records a pair of HIP events around its streaming filter
+ * kernel on the call's stream.  rk_timing_read waits for them, returns the
+ * summed kernel milliseconds and the number of launches timed, and clears. */
+int rk_timing_enable(int on);
+int rk_timing_read(double* filter_ms, int32_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RADIK_AMD_H_ */

@@ -1,0 +1,285 @@
+"""radik_amd — MI355X-native radix top-k (RadiK's batched top-k path).
+
+Host-side mirror of the reference interface over the C-ABI library
+``radik_amd/lib/libradik_amd.so`` (hand-written HIP for gfx950):
+
+* :func:`topk_radix_select` mirrors ``topKRadixSelectL``
+  (radik/RadixSelect/topk_radixselect.h:59-323): host ``taskLen``, one ``K``,
+  output ``[taskNum][K]`` with the tail fill (index -1, value 0).
+* :func:`get_workspace_size` mirrors ``getRadixSelectLWorkSpaceSize``
+  (topk_radixselect.h:22-33).
+* :func:`topk_batched` is the uneven-batch form: device row offsets, per-row
+  k, per-row output offsets.
+
+PyTorch is used only for device memory and streams.  There is no CPU
+fallback: every call goes through the HIP library and raises if it is not
+built or no GPU is present.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+__all__ = [
+    "RK_F32", "RK_F16", "RK_BF16", "RK_LARGEST", "RK_SORTED", "RK_ASCEND", "RK_EXACT",
+    "RadikError", "lib", "lib_path", "max_k", "get_workspace_size", "workspace_size",
+    "topk_radix_select", "topk_batched", "topk_rows", "status", "gen_fill",
+    "get_tune", "set_tune", "timing_enable", "timing_read",
+]
+
+RK_F32, RK_F16, RK_BF16 = 0, 1, 2
+RK_LARGEST, RK_SORTED, RK_ASCEND, RK_EXACT = 1, 2, 4, 8
+GEN_UNIFORM, GEN_NORMAL, GEN_ZIPF, GEN_NEARCONST, GEN_RANGE, GEN_CONST = range(6)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "lib", "libradik_amd.so")
+_lib = None
+
+_ERRORS = {
+    0: "ok", -1: "bad argument", -2: "workspace missing or too small",
+    -3: "unsupported dtype or k above rk_max_k()", -4: "misaligned pointer",
+    -5: "HIP runtime error", -6: "device-side validation failed",
+}
+
+# Every symbol include/radik_amd.h and include/radik_amd_harness.h declare.
+EXPORTED = (
+    "rk_max_k", "rk_topk_workspace_size", "rk_topk_batched", "rk_topk_rows",
+    "rk_compat_workspace_size", "rk_topk_compat", "rk_topk_status", "rk_strerror",
+    "rk_get_tune", "rk_set_tune", "rk_gen_fill", "rk_timing_enable", "rk_timing_read",
+)
+
+
+class RadikError(RuntimeError):
+    """A radik_amd C-ABI call returned an error code."""
+
+    def __init__(self, code: int, where: str):
+        super().__init__(f"{where}: {_ERRORS.get(code, 'unknown error')} (code {code})")
+        self.code = code
+
+
+class _Tune(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in
+                ("tile", "grid", "sample_groups", "group_elems", "hist_bits", "unroll")]
+
+
+def lib_path() -> str:
+    return _LIB_PATH
+
+
+def lib():
+    """Loads the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise RuntimeError(
+            f"radik_amd: {_LIB_PATH} is missing; build it with `python -m radik_amd.build`")
+    L = ctypes.CDLL(_LIB_PATH)
+    c = ctypes
+    vp, i32, i64, u32, sz = c.c_void_p, c.c_int32, c.c_int64, c.c_uint32, c.c_size_t
+    L.rk_max_k.restype = i32
+    L.rk_topk_workspace_size.argtypes = [c.c_int, i32, i64, i32, u32, c.POINTER(sz)]
+    L.rk_topk_batched.argtypes = [c.c_int, vp, vp, vp, i32, i64, i32, u32, vp, vp, vp, vp,
+                                  vp, sz, vp]
+    L.rk_topk_rows.argtypes = [c.c_int, vp, vp, vp, i32, i64, i32, u32, vp, vp, vp, vp, sz, vp]
+    L.rk_compat_workspace_size.argtypes = [c.c_int, i32, i32, i32, u32, c.POINTER(sz)]
+    L.rk_topk_compat.argtypes = [c.c_int, vp, vp, vp, vp, vp, sz, vp, i32, i32, u32, vp]
+    L.rk_topk_status.argtypes = [vp, vp, c.POINTER(u32), c.POINTER(u32), c.POINTER(u32)]
+    L.rk_strerror.restype = c.c_char_p
+    L.rk_strerror.argtypes = [c.c_int]
+    L.rk_get_tune.argtypes = [c.POINTER(_Tune)]
+    L.rk_set_tune.argtypes = [c.POINTER(_Tune)]
+    L.rk_gen_fill.argtypes = [c.c_int, c.c_int, c.c_uint64, c.c_uint64, i64, i64, c.c_double,
+                              c.c_double, c.c_double, vp, vp]
+    L.rk_timing_enable.argtypes = [c.c_int]
+    L.rk_timing_read.argtypes = [c.POINTER(c.c_double), c.POINTER(i32)]
+    for name in ("rk_timing_enable", "rk_timing_read",
+                 "rk_topk_workspace_size", "rk_topk_batched", "rk_topk_rows",
+                 "rk_compat_workspace_size", "rk_topk_compat", "rk_topk_status",
+                 "rk_get_tune", "rk_set_tune", "rk_gen_fill"):
+        getattr(L, name).restype = c.c_int
+    _lib = L
+    return L
+
+
+def _check(rc: int, where: str) -> None:
+    if rc != 0:
+        raise RadikError(rc, where)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _dtype_code(t) -> int:
+    torch = _torch()
+    if t.dtype == torch.float32:
+        return RK_F32
+    if t.dtype == torch.float16:
+        return RK_F16
+    if t.dtype == torch.bfloat16:
+        return RK_BF16
+    raise TypeError(f"radik_amd: unsupported dtype {t.dtype}")
+
+
+def _stream_ptr(stream=None) -> int:
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def _ptr(t):
+    return None if t is None else t.data_ptr()
+
+
+def _flags(largest: bool, sorted: bool, ascend: bool | None, exact: bool) -> int:
+    if ascend is None:  # torch.topk order: best first
+        ascend = not largest
+    return ((RK_LARGEST if largest else 0) | (RK_SORTED if sorted else 0)
+            | (RK_ASCEND if ascend else 0) | (RK_EXACT if exact else 0))
+
+
+def max_k() -> int:
+    return int(lib().rk_max_k())
+
+
+def workspace_size(dtype: int, num_rows: int, total_len: int, max_k_: int, flags: int = 0) -> int:
+    out = ctypes.c_size_t(0)
+    _check(lib().rk_topk_workspace_size(dtype, num_rows, total_len, max_k_, flags,
+                                        ctypes.byref(out)), "rk_topk_workspace_size")
+    return int(out.value)
+
+
+def get_workspace_size(K: int, n: int, taskNum: int, dtype: int = RK_F32) -> int:
+    """Mirror of getRadixSelectLWorkSpaceSize<ValType>(K, n, taskNum, &bytes)."""
+    out = ctypes.c_size_t(0)
+    _check(lib().rk_compat_workspace_size(dtype, K, n, taskNum, 0, ctypes.byref(out)),
+           "rk_compat_workspace_size")
+    return int(out.value)
+
+
+def _alloc_ws(nbytes: int, device):
+    torch = _torch()
+    return torch.empty(max(nbytes, 256), dtype=torch.uint8, device=device)
+
+
+def topk_radix_select(valIn, K: int, taskLen, *, idxIn=None, largest: bool = False,
+                      ascend: bool = True, need_sorting: bool = True, exact: bool = False,
+                      workspace=None, valOut=None, idxOut=None, stream=None):
+    """Mirror of topKRadixSelectL<IdxType, LARGEST, ASCEND, ...>(valIn, idxIn,
+    valOut, idxOut, workSpace, taskLen, taskNum, K, stream, needSorting).
+
+    The reference's defaults in its harness are smallest-k, ascending
+    (radik/test/test_radik.cu:18-20); they are kept here.  Returns
+    ``(valOut, idxOut)`` shaped ``[taskNum, K]``.  WITHSCALE has no switch:
+    the key-space scaling is always exact and always on.
+    """
+    torch = _torch()
+    taskLen = [int(x) for x in taskLen]
+    taskNum = len(taskLen)
+    dt = _dtype_code(valIn)
+    if valOut is None:
+        valOut = torch.empty((taskNum, K), dtype=valIn.dtype, device=valIn.device)
+    if idxOut is None:
+        idxOut = torch.empty((taskNum, K), dtype=torch.int32, device=valIn.device)
+    if workspace is None:
+        workspace = _alloc_ws(get_workspace_size(K, max(taskLen, default=0), taskNum, dt),
+                              valIn.device)
+    tl = (ctypes.c_int32 * max(taskNum, 1))(*taskLen)
+    flags = _flags(largest, need_sorting, ascend, exact)
+    rc = lib().rk_topk_compat(dt, _ptr(valIn), _ptr(idxIn), _ptr(valOut), _ptr(idxOut),
+                              _ptr(workspace), workspace.numel(), tl, taskNum, K, flags,
+                              _stream_ptr(stream))
+    _check(rc, "rk_topk_compat")
+    return valOut, idxOut
+
+
+def topk_batched(values, row_off, row_k, *, largest: bool = True, sorted: bool = True,
+                 ascend: bool | None = None, exact: bool = False, idx_in=None, out_off=None,
+                 total_len: int | None = None, max_k_: int | None = None, workspace=None,
+                 out_vals=None, out_idx=None, stream=None):
+    """Uneven batched top-k.  ``values`` 1-D device tensor of concatenated rows,
+    ``row_off`` int64 [R+1] and ``row_k`` int32 [R] on the device.  Output row
+    r occupies ``[out_off[r], out_off[r+1])`` (default: packed by min(len, k)).
+    Host bounds ``total_len`` / ``max_k_`` default to the exact values (which
+    reads the metadata back; pass them to stay asynchronous)."""
+    torch = _torch()
+    dt = _dtype_code(values)
+    R = row_k.numel()
+    if total_len is None:
+        total_len = int(row_off[-1].item() - row_off[0].item()) if R else 0
+    if max_k_ is None:
+        max_k_ = int(row_k.max().item()) if R else 0
+    if out_off is None:
+        lens = (row_off[1:] - row_off[:-1])
+        m = torch.minimum(lens, row_k.to(torch.int64))
+        out_off = torch.zeros(R + 1, dtype=torch.int64, device=values.device)
+        out_off[1:] = torch.cumsum(m, 0)
+    n_out = int(out_off[-1].item()) if out_vals is None or out_idx is None else 0
+    if out_vals is None:
+        out_vals = torch.empty(max(n_out, 1), dtype=values.dtype, device=values.device)
+    if out_idx is None:
+        out_idx = torch.empty(max(n_out, 1), dtype=torch.int32, device=values.device)
+    flags = _flags(largest, sorted, ascend, exact)
+    if workspace is None:
+        workspace = _alloc_ws(workspace_size(dt, R, total_len, max_k_, flags), values.device)
+    rc = lib().rk_topk_batched(dt, _ptr(values), _ptr(row_off), _ptr(row_k), R, total_len,
+                               max_k_, flags, _ptr(idx_in), _ptr(out_vals), _ptr(out_idx),
+                               _ptr(out_off), _ptr(workspace), workspace.numel(),
+                               _stream_ptr(stream))
+    _check(rc, "rk_topk_batched")
+    return out_vals, out_idx, out_off, workspace
+
+
+def topk_rows(row_ptrs, row_len, row_k, out_off, out_vals, out_idx, *, dtype: int, total_len: int,
+              max_k_: int, largest: bool = True, sorted: bool = True, ascend: bool | None = None,
+              exact: bool = False, workspace, stream=None) -> None:
+    """Pointer form: rows at arbitrary device addresses (device int64 pointer array)."""
+    flags = _flags(largest, sorted, ascend, exact)
+    rc = lib().rk_topk_rows(dtype, _ptr(row_ptrs), _ptr(row_len), _ptr(row_k), row_k.numel(),
+                            total_len, max_k_, flags, _ptr(out_vals), _ptr(out_idx),
+                            _ptr(out_off), _ptr(workspace), workspace.numel(),
+                            _stream_ptr(stream))
+    _check(rc, "rk_topk_rows")
+
+
+def status(workspace, stream=None):
+    """(status, fallback_rows, ordered_rows) of the last call on ``workspace``."""
+    s, f, o = ctypes.c_uint32(0), ctypes.c_uint32(0), ctypes.c_uint32(0)
+    rc = lib().rk_topk_status(_ptr(workspace), _stream_ptr(stream), ctypes.byref(s),
+                              ctypes.byref(f), ctypes.byref(o))
+    if rc not in (0, -6):
+        _check(rc, "rk_topk_status")
+    return int(s.value), int(f.value), int(o.value)
+
+
+def gen_fill(out, kind: int, seed: int, stream_id: int = 0, *, a: float = 0.0, b: float = 1.0,
+             dup_frac: float = 0.0, row_len: int = 0, stream=None) -> None:
+    """Seeded device-side input generation (harness; see radik_amd_harness.h)."""
+    rc = lib().rk_gen_fill(_dtype_code(out), kind, seed, stream_id, out.numel(), row_len, a, b,
+                           dup_frac, _ptr(out), _stream_ptr(stream))
+    _check(rc, "rk_gen_fill")
+
+
+def timing_enable(on: bool = True) -> None:
+    _check(lib().rk_timing_enable(1 if on else 0), "rk_timing_enable")
+
+
+def timing_read():
+    """(summed filter-kernel milliseconds, launches timed) since the last read."""
+    ms, n = ctypes.c_double(0.0), ctypes.c_int32(0)
+    _check(lib().rk_timing_read(ctypes.byref(ms), ctypes.byref(n)), "rk_timing_read")
+    return float(ms.value), int(n.value)
+
+
+def get_tune() -> dict:
+    t = _Tune()
+    _check(lib().rk_get_tune(ctypes.byref(t)), "rk_get_tune")
+    return {n: getattr(t, n) for n, _ in _Tune._fields_}
+
+
+def set_tune(**kw) -> None:
+    t = _Tune(**{n: int(kw.get(n, 0)) for n, _ in _Tune._fields_})
+    _check(lib().rk_set_tune(ctypes.byref(t)), "rk_set_tune")

@@ -1,0 +1,306 @@
+// C-ABI layer: argument validation, workspace layout, device limits and the
+// reference-surface compat entry.  Launches go through rk::run_pipeline.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/radik_amd.h"
+#include "rk_pipeline.hpp"
+
+namespace rk {
+hipError_t run_pipeline(int dtype, const KParams& p, hipStream_t stream, int row_grid, int grid);
+int lds_keys_max();
+int timing_enable(int on);
+int timing_read(double* ms, int32_t* launches);
+}  // namespace rk
+
+namespace {
+
+using rk::KParams;
+using rk::Tune;
+
+std::mutex g_mu;
+Tune g_tune{16384, 0, 1024, 16, 11, 4};
+
+struct DevInfo {
+  int cus = 0;
+  int lds_keys = 0;
+  bool ok = false;
+};
+
+DevInfo device_info() {
+  static std::mutex mu;
+  static std::vector<DevInfo> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return DevInfo{};
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int)cache.size() <= dev) cache.resize(dev + 1);
+  DevInfo& d = cache[dev];
+  if (!d.ok) {
+    int cus = 0, lds = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return DevInfo{};
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      return DevInfo{};
+    // LDS of the select kernels: the key buffer plus ~10 KB of static arrays.
+    int keys = rk::lds_keys_max();
+    while (keys > 1024 && (size_t)keys * 8 + 10240 > (size_t)lds) keys >>= 1;
+    d.cus = cus;
+    d.lds_keys = keys;
+    d.ok = true;
+  }
+  return d;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct Layout {
+  size_t hdr, desc, st, tile_start, cand_base, cand_cnt, ghist, tile_cnt, tile_pre, cand, total;
+  uint64_t tile_cap, cand_cap;
+};
+
+// Candidate slots of a row: min(len, 4k + ceil(len/64) + 1024) (set on the
+// device by k_plan_rows); summed over rows this is bounded by the expression
+// below, which needs only the batch totals.
+Layout make_layout(int32_t R, int64_t total_len, int32_t max_k) {
+  Layout L{};
+  const uint64_t Ru = (uint64_t)R;
+  const uint64_t E = (uint64_t)total_len;
+  L.tile_cap = E / rk::TILE_MIN + Ru + 1;
+  const uint64_t cand_bound = 4 * Ru * (uint64_t)max_k + E / 64 + 1025 * Ru;
+  L.cand_cap = std::min<uint64_t>(E, cand_bound);
+  size_t o = 0;
+  L.hdr = o; o = align256(o + sizeof(rk::Header));
+  L.desc = o; o = align256(o + Ru * sizeof(rk::RowDesc));
+  L.st = o; o = align256(o + Ru * sizeof(rk::RowState));
+  L.tile_start = o; o = align256(o + (Ru + 1) * sizeof(uint32_t));
+  L.cand_base = o; o = align256(o + (Ru + 1) * sizeof(uint64_t));
+  L.cand_cnt = o; o = align256(o + Ru * sizeof(uint32_t));
+  L.ghist = o; o = align256(o + Ru * rk::HIST_MAX_BINS * sizeof(uint32_t));
+  L.tile_cnt = o; o = align256(o + L.tile_cap * sizeof(uint2));
+  L.tile_pre = o; o = align256(o + L.tile_cap * sizeof(uint2));
+  L.cand = o; o = align256(o + L.cand_cap * sizeof(uint64_t));
+  L.total = o;
+  return L;
+}
+
+int elem_size(int dtype) { return dtype == RK_F32 ? 4 : 2; }
+
+bool valid_dtype(int dtype) { return dtype == RK_F32 || dtype == RK_F16 || dtype == RK_BF16; }
+
+Tune current_tune(const DevInfo& di) {
+  Tune t;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    t = g_tune;
+  }
+  if (t.tile < rk::TILE_MIN) t.tile = rk::TILE_MIN;
+  t.tile = (t.tile / 64) * 64;
+  if (t.grid <= 0) t.grid = di.cus * 8;
+  if (t.sample_groups <= 0) t.sample_groups = 1024;
+  if (t.group_elems <= 0) t.group_elems = 16;
+  t.group_elems = std::max(8, (t.group_elems / 8) * 8);
+  if (t.hist_bits < 8 || t.hist_bits > 11) t.hist_bits = 11;
+  if (t.unroll != 2 && t.unroll != 4 && t.unroll != 8) t.unroll = 4;
+  return t;
+}
+
+// Shared body of the batched entry points.
+int launch(int dtype, const void* vals, const int64_t* row_off, const void* const* row_ptrs,
+           const int64_t* row_len, const int32_t* row_k, int32_t K, int32_t num_rows,
+           int64_t total_len, int32_t max_k, uint32_t flags, const int32_t* idx_in,
+           void* out_vals, int32_t* out_idx, const int64_t* out_off, void* workspace,
+           size_t ws_bytes, void* stream) {
+  if (!valid_dtype(dtype)) return RK_E_UNSUPPORTED;
+  if (num_rows < 0 || total_len < 0 || max_k < 0) return RK_E_ARG;
+  if (num_rows == 0) return RK_OK;
+  if (!out_vals || !out_idx) return RK_E_ARG;
+  if (row_ptrs ? !row_len : (!vals || !row_off)) return RK_E_ARG;
+  const DevInfo di = device_info();
+  if (!di.ok) return RK_E_HIP;
+  if (max_k > di.lds_keys) return RK_E_UNSUPPORTED;
+  const int es = elem_size(dtype);
+  if (((uintptr_t)vals % es) || ((uintptr_t)out_vals % es)) return RK_E_ALIGN;
+  if (!workspace) return RK_E_WORKSPACE;
+  if ((uintptr_t)workspace % 256) return RK_E_ALIGN;
+  const Layout L = make_layout(num_rows, total_len, max_k);
+  if (ws_bytes < L.total) return RK_E_WORKSPACE;
+
+  char* ws = static_cast<char*>(workspace);
+  KParams p{};
+  p.vals = vals;
+  p.row_off = row_off;
+  p.row_k = row_k;
+  p.out_off = out_off;
+  p.idx_in = idx_in;
+  p.row_ptrs = row_ptrs;
+  p.row_len = row_len;
+  p.out_vals = out_vals;
+  p.out_idx = out_idx;
+  p.K = K;
+  p.R = num_rows;
+  p.max_k = max_k;
+  p.flags = flags;
+  p.total_len = total_len;
+  p.hdr = reinterpret_cast<rk::Header*>(ws + L.hdr);
+  p.desc = reinterpret_cast<rk::RowDesc*>(ws + L.desc);
+  p.st = reinterpret_cast<rk::RowState*>(ws + L.st);
+  p.tile_start = reinterpret_cast<uint32_t*>(ws + L.tile_start);
+  p.cand_base = reinterpret_cast<uint64_t*>(ws + L.cand_base);
+  p.cand_cnt = reinterpret_cast<uint32_t*>(ws + L.cand_cnt);
+  p.ghist = reinterpret_cast<uint32_t*>(ws + L.ghist);
+  p.tile_cnt = reinterpret_cast<uint2*>(ws + L.tile_cnt);
+  p.tile_pre = reinterpret_cast<uint2*>(ws + L.tile_pre);
+  p.cand = reinterpret_cast<uint64_t*>(ws + L.cand);
+  p.tile_capacity = L.tile_cap;
+  p.cand_capacity = L.cand_cap;
+  p.lds_keys = di.lds_keys;
+  p.small_max = di.lds_keys;
+  p.tune = current_tune(di);
+  const int row_grid = std::min(num_rows, 8192);
+  const uint64_t tiles_bound = (uint64_t)total_len / (uint64_t)p.tune.tile + (uint64_t)num_rows;
+  const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)p.tune.grid, tiles_bound));
+  const hipError_t e = rk::run_pipeline(dtype, p, static_cast<hipStream_t>(stream), row_grid, grid);
+  return e == hipSuccess ? RK_OK : RK_E_HIP;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t rk_max_k(void) {
+  const DevInfo di = device_info();
+  return di.ok ? di.lds_keys : rk::lds_keys_max();
+}
+
+int rk_topk_workspace_size(int dtype, int32_t num_rows, int64_t total_len, int32_t max_k,
+                           uint32_t flags, size_t* bytes) {
+  (void)flags;
+  if (!bytes) return RK_E_ARG;
+  if (!valid_dtype(dtype)) return RK_E_UNSUPPORTED;
+  if (num_rows < 0 || total_len < 0 || max_k < 0) return RK_E_ARG;
+  *bytes = make_layout(num_rows, total_len, max_k).total;
+  return RK_OK;
+}
+
+int rk_topk_batched(int dtype, const void* vals, const int64_t* row_off, const int32_t* row_k,
+                    int32_t num_rows, int64_t total_len, int32_t max_k, uint32_t flags,
+                    const int32_t* idx_in, void* out_vals, int32_t* out_idx,
+                    const int64_t* out_off, void* workspace, size_t ws_bytes, void* stream) {
+  if (!valid_dtype(dtype)) return RK_E_UNSUPPORTED;
+  if (!row_k || !out_off) return RK_E_ARG;
+  return launch(dtype, vals, row_off, nullptr, nullptr, row_k, 0, num_rows, total_len, max_k,
+                flags, idx_in, out_vals, out_idx, out_off, workspace, ws_bytes, stream);
+}
+
+int rk_topk_rows(int dtype, const void* const* row_ptrs, const int64_t* row_len,
+                 const int32_t* row_k, int32_t num_rows, int64_t total_len, int32_t max_k,
+                 uint32_t flags, void* out_vals, int32_t* out_idx, const int64_t* out_off,
+                 void* workspace, size_t ws_bytes, void* stream) {
+  if (!valid_dtype(dtype)) return RK_E_UNSUPPORTED;
+  if (!row_ptrs || !row_len || !row_k || !out_off) return RK_E_ARG;
+  return launch(dtype, nullptr, nullptr, row_ptrs, row_len, row_k, 0, num_rows, total_len,
+                max_k, flags, nullptr, out_vals, out_idx, out_off, workspace, ws_bytes, stream);
+}
+
+int rk_compat_workspace_size(int dtype, int32_t K, int32_t n, int32_t taskNum, uint32_t flags,
+                             size_t* bytes) {
+  if (!bytes) return RK_E_ARG;
+  if (K < 0 || n < 0 || taskNum < 0) return RK_E_ARG;
+  size_t b = 0;
+  const int rc = rk_topk_workspace_size(dtype, taskNum, (int64_t)n * taskNum, K, flags, &b);
+  if (rc != RK_OK) return rc;
+  *bytes = b + align256(((size_t)taskNum + 1) * sizeof(int64_t));
+  return RK_OK;
+}
+
+int rk_topk_compat(int dtype, const void* valIn, const int32_t* idxIn, void* valOut,
+                   int32_t* idxOut, void* workspace, size_t ws_bytes, const int32_t* taskLen,
+                   int32_t taskNum, int32_t K, uint32_t flags, void* stream) {
+  if (!taskLen || taskNum < 0 || K < 0) return RK_E_ARG;
+  if (!workspace) return RK_E_WORKSPACE;  // topk_radixselect.h:77-79
+  if (taskNum == 0) return RK_OK;
+  std::vector<int64_t> off((size_t)taskNum + 1, 0);
+  int32_t longest = 0;
+  for (int32_t i = 0; i < taskNum; ++i) {
+    if (taskLen[i] < 0) return RK_E_ARG;
+    off[i + 1] = off[i] + taskLen[i];
+    longest = std::max(longest, taskLen[i]);
+  }
+  size_t need = 0;
+  int rc = rk_compat_workspace_size(dtype, K, longest, taskNum, flags, &need);
+  if (rc != RK_OK) return rc;
+  if (ws_bytes < need) return RK_E_WORKSPACE;
+  const size_t off_bytes = align256(((size_t)taskNum + 1) * sizeof(int64_t));
+  char* ws = static_cast<char*>(workspace);
+  int64_t* d_off = reinterpret_cast<int64_t*>(ws);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (hipMemcpyAsync(d_off, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice, s) !=
+      hipSuccess)
+    return RK_E_HIP;
+  return launch(dtype, valIn, d_off, nullptr, nullptr, nullptr, K, taskNum,
+                (int64_t)longest * taskNum, K, flags, idxIn, valOut, idxOut, nullptr,
+                ws + off_bytes, ws_bytes - off_bytes, stream);
+}
+
+int rk_topk_status(const void* workspace, void* stream, uint32_t* status, uint32_t* fallback_rows,
+                   uint32_t* ordered_rows) {
+  if (!workspace) return RK_E_WORKSPACE;
+  rk::Header h{};
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (hipMemcpyAsync(&h, workspace, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess)
+    return RK_E_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess) return RK_E_HIP;
+  if (status) *status = h.status;
+  if (fallback_rows) *fallback_rows = h.fallback_rows;
+  if (ordered_rows) *ordered_rows = h.ordered_rows;
+  return h.status ? RK_E_DEVICE : RK_OK;
+}
+
+const char* rk_strerror(int code) {
+  switch (code) {
+    case RK_OK: return "ok";
+    case RK_E_ARG: return "bad argument";
+    case RK_E_WORKSPACE: return "workspace missing or too small";
+    case RK_E_UNSUPPORTED: return "unsupported dtype or k above rk_max_k()";
+    case RK_E_ALIGN: return "pointer not aligned to its element size (workspace: 256 bytes)";
+    case RK_E_HIP: return "HIP runtime error";
+    case RK_E_DEVICE: return "device-side validation failed";
+    default: return "unknown error";
+  }
+}
+
+int rk_timing_enable(int on) { return rk::timing_enable(on) == 0 ? RK_OK : RK_E_HIP; }
+
+int rk_timing_read(double* filter_ms, int32_t* launches) {
+  return rk::timing_read(filter_ms, launches) == 0 ? RK_OK : RK_E_HIP;
+}
+
+int rk_get_tune(rk_tune* out) {
+  if (!out) return RK_E_ARG;
+  std::lock_guard<std::mutex> lk(g_mu);
+  out->tile = g_tune.tile;
+  out->grid = g_tune.grid;
+  out->sample_groups = g_tune.sample_groups;
+  out->group_elems = g_tune.group_elems;
+  out->hist_bits = g_tune.hist_bits;
+  out->unroll = g_tune.unroll;
+  return RK_OK;
+}
+
+int rk_set_tune(const rk_tune* in) {
+  if (!in) return RK_E_ARG;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (in->tile > 0) g_tune.tile = in->tile;
+  if (in->grid > 0) g_tune.grid = in->grid;
+  if (in->sample_groups > 0) g_tune.sample_groups = in->sample_groups;
+  if (in->group_elems > 0) g_tune.group_elems = in->group_elems;
+  if (in->hist_bits > 0) g_tune.hist_bits = in->hist_bits;
+  if (in->unroll > 0) g_tune.unroll = in->unroll;
+  return RK_OK;
+}
+
+}  // extern "C"

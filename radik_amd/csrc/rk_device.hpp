@@ -1,0 +1,310 @@
+// Device-side building blocks for the gfx950 radix top-k: element traits,
+// the order-preserving key flip, wave64 / workgroup scans and reductions,
+// the workgroup radix select over keys, and the LDS bitonic sort.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define RK_DEV __device__ __forceinline__
+
+namespace rk {
+
+enum Dtype : int { F32 = 0, F16 = 1, BF16 = 2 };
+
+template <int DT>
+struct Traits;
+template <>
+struct Traits<F32> {
+  using U = uint32_t;                 // storage word of one element
+  static constexpr int KEY_BITS = 32; // key width
+  static constexpr int PACK = 4;      // elements per 16-byte load
+  static constexpr uint32_t KEY_MASK = 0xffffffffu;
+};
+template <>
+struct Traits<F16> {
+  using U = uint16_t;
+  static constexpr int KEY_BITS = 16;
+  static constexpr int PACK = 8;
+  static constexpr uint32_t KEY_MASK = 0xffffu;
+};
+template <>
+struct Traits<BF16> {
+  using U = uint16_t;
+  static constexpr int KEY_BITS = 16;
+  static constexpr int PACK = 8;
+  static constexpr uint32_t KEY_MASK = 0xffffu;
+};
+
+// ---------------------------------------------------------------------------
+// Keys.  The flip follows radik/RadixSelect/utils.cuh:329-334 (mask = sign ?
+// all-ones : sign-bit; key = bits ^ mask), applied at the element's native
+// width (32 bits for f32, 16 for f16/bf16).  The selection key makes "larger
+// is better" for both directions (bit-inverted for smallest-k) and sends NaN
+// to 0, the worst key, so NaN is chosen only when fewer than k non-NaN exist.
+// ---------------------------------------------------------------------------
+template <int DT, bool LARGEST>
+RK_DEV uint32_t sel_key(uint32_t u) {
+  if constexpr (DT == F32) {
+    uint32_t f = u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u);
+    if (!LARGEST) f = ~f;
+    return ((u & 0x7fffffffu) > 0x7f800000u) ? 0u : f;
+  } else {
+    constexpr uint32_t NAN_ABS = (DT == F16) ? 0x7c00u : 0x7f80u;
+    uint32_t f = (u ^ ((u & 0x8000u) ? 0xffffu : 0x8000u)) & 0xffffu;
+    if (!LARGEST) f ^= 0xffffu;
+    return ((u & 0x7fffu) > NAN_ABS) ? 0u : f;
+  }
+}
+
+// Inverse of sel_key for a non-zero key: the element's bit pattern.
+template <int DT, bool LARGEST>
+RK_DEV uint32_t key_to_bits(uint32_t sk) {
+  if constexpr (DT == F32) {
+    uint32_t f = LARGEST ? sk : ~sk;
+    return (f & 0x80000000u) ? (f ^ 0x80000000u) : ~f;
+  } else {
+    uint32_t f = LARGEST ? sk : (sk ^ 0xffffu);
+    return (f & 0x8000u) ? (f ^ 0x8000u) : (~f & 0xffffu);
+  }
+}
+
+template <int DT>
+RK_DEV float bits_to_float(uint32_t u) {
+  if constexpr (DT == F32) {
+    return __uint_as_float(u);
+  } else if constexpr (DT == BF16) {
+    return __uint_as_float(u << 16);
+  } else {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)u);
+  }
+}
+
+template <int DT>
+RK_DEV bool bits_is_nan(uint32_t u) {
+  if constexpr (DT == F32) return (u & 0x7fffffffu) > 0x7f800000u;
+  else if constexpr (DT == F16) return (u & 0x7fffu) > 0x7c00u;
+  else return (u & 0x7fffu) > 0x7f80u;
+}
+
+// Float threshold for a key threshold T > 0, such that for every non-NaN x:
+//   sel_key(x) >= T  implies  (LARGEST ? x >= tf : x <= tf).
+// (The converse can fail only for -0/+0, which only adds candidates.)
+template <int DT, bool LARGEST>
+RK_DEV float key_threshold_float(uint32_t T) {
+  constexpr uint32_t HALF = 1u << (Traits<DT>::KEY_BITS - 1);
+  const uint32_t b = key_to_bits<DT, LARGEST>(T);
+  if (bits_is_nan<DT>(b)) {
+    // T lies in the part of key space no non-NaN value occupies.
+    const bool low = T < HALF;
+    const float inf = __builtin_inff();
+    return low ? (LARGEST ? -inf : inf) : (LARGEST ? inf : -inf);
+  }
+  return bits_to_float<DT>(b);
+}
+
+// Unpacked 16-byte vector of PACK elements (bit patterns, low element first).
+template <int DT>
+struct Vec {
+  uint32_t e[Traits<DT>::PACK];
+};
+
+template <int DT>
+RK_DEV Vec<DT> unpack(const uint4& x) {
+  Vec<DT> v;
+  if constexpr (DT == F32) {
+    v.e[0] = x.x; v.e[1] = x.y; v.e[2] = x.z; v.e[3] = x.w;
+  } else {
+    v.e[0] = x.x & 0xffffu; v.e[1] = x.x >> 16;
+    v.e[2] = x.y & 0xffffu; v.e[3] = x.y >> 16;
+    v.e[4] = x.z & 0xffffu; v.e[5] = x.z >> 16;
+    v.e[6] = x.w & 0xffffu; v.e[7] = x.w >> 16;
+  }
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Wave64 / workgroup primitives.
+// ---------------------------------------------------------------------------
+RK_DEV int lane_id() { return threadIdx.x & 63; }
+
+// popcount(mask & lanes below this one)
+RK_DEV uint32_t mbcnt(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+RK_DEV uint32_t wave_incl_scan(uint32_t x) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+
+RK_DEV uint32_t wave_sum(uint32_t x) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+  return x;
+}
+
+RK_DEV uint64_t wave_or64(uint64_t x) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) x |= __shfl_xor(x, d, 64);
+  return x;
+}
+
+// Exclusive scan over the workgroup in thread order.  scratch: >= NT/64 words.
+template <int NT>
+RK_DEV uint32_t block_excl_scan(uint32_t x, uint32_t* scratch, uint32_t& total) {
+  constexpr int NW = NT / 64;
+  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  const uint32_t inc = wave_incl_scan(x);
+  if (lane == 63) scratch[wid] = inc;
+  __syncthreads();
+  uint32_t pre = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    const uint32_t v = scratch[w];
+    pre += (w < wid) ? v : 0u;
+    tot += v;
+  }
+  __syncthreads();
+  total = tot;
+  return pre + inc - x;
+}
+
+template <int NT>
+RK_DEV uint32_t block_sum(uint32_t x, uint32_t* scratch) {
+  constexpr int NW = NT / 64;
+  x = wave_sum(x);
+  if (lane_id() == 0) scratch[threadIdx.x >> 6] = x;
+  __syncthreads();
+  uint32_t tot = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) tot += scratch[w];
+  __syncthreads();
+  return tot;
+}
+
+template <int NT>
+RK_DEV uint64_t block_or64(uint64_t x, uint64_t* scratch) {
+  constexpr int NW = NT / 64;
+  x = wave_or64(x);
+  if (lane_id() == 0) scratch[threadIdx.x >> 6] = x;
+  __syncthreads();
+  uint64_t r = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) r |= scratch[w];
+  __syncthreads();
+  return r;
+}
+
+// Finds, scanning from the top bin down, the bin b with
+//   above(b) < k <= above(b) + hist[b]
+// where above(b) = sum of hist over bins > b.  Result in out[0..2] =
+// {b, above(b), hist[b]}; out[0] = 0xffffffff if sum(hist) < k.
+// Thread t owns a contiguous run of bins counted from the top, so an
+// exclusive prefix scan over threads yields above().
+template <int NT>
+RK_DEV void block_find_crossing(const uint32_t* hist, int nbins, uint32_t k,
+                                uint32_t* scratch, uint32_t* out) {
+  if (threadIdx.x == 0) out[0] = 0xffffffffu;
+  const int per = (nbins + NT - 1) / NT;
+  const int hi = nbins - (int)threadIdx.x * per;
+  uint32_t s = 0;
+  for (int j = 0; j < per; ++j) {
+    const int b = hi - 1 - j;
+    if (b >= 0) s += hist[b];
+  }
+  uint32_t total;
+  const uint32_t above = block_excl_scan<NT>(s, scratch, total);  // has barriers
+  if (s != 0 && above < k && k <= above + s) {
+    uint32_t a = above;
+    for (int j = 0; j < per; ++j) {
+      const int b = hi - 1 - j;
+      if (b < 0) break;
+      const uint32_t c = hist[b];
+      if (k <= a + c) {
+        out[0] = (uint32_t)b;
+        out[1] = a;
+        out[2] = c;
+        break;
+      }
+      a += c;
+    }
+  }
+  __syncthreads();
+}
+
+// Workgroup radix select: returns a threshold K such that
+//   #{i : keys[i] >= K} >= k  and  #{i : keys[i] > K'} < k for K' >= K in
+// the selected bin, i.e. keys >= K are exactly the k largest when the keys
+// are distinct (composite keys), and at least the k largest otherwise.
+// `keys` may point to LDS or global memory.  hist: LDS, 1 << HB words.
+// The scan starts below the common prefix of all keys (the key-space form
+// of RadiK's adaptive scaling): bits every key shares are never histogrammed.
+template <int NT, int HB, typename KeyT>
+RK_DEV KeyT block_radix_select(const KeyT* keys, uint32_t n, uint32_t k,
+                               uint32_t* hist, uint32_t* scratch,
+                               uint64_t* scratch64, uint32_t* out) {
+  constexpr int BITS = (int)sizeof(KeyT) * 8;
+  const KeyT k0 = keys[0];
+  uint64_t diff = 0;
+  for (uint32_t i = threadIdx.x; i < n; i += NT) diff |= (uint64_t)(keys[i] ^ k0);
+  diff = block_or64<NT>(diff, scratch64);
+  if (diff == 0) return k0;  // all keys equal: every key is >= k0
+  int lo = 64 - __builtin_clzll(diff);  // number of low bits still unresolved
+  KeyT prefix = (lo >= BITS) ? (KeyT)0 : (KeyT)((k0 >> lo) << lo);
+  uint32_t rem = k;
+  while (lo > 0) {
+    const int w = lo < HB ? lo : HB;
+    const int shift = lo - w;
+    const uint32_t nb = 1u << w;
+    for (uint32_t i = threadIdx.x; i < nb; i += NT) hist[i] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += NT) {
+      const KeyT x = keys[i];
+      const bool match = (lo >= BITS) || ((x >> lo) == (prefix >> lo));
+      if (match) atomicAdd(&hist[(uint32_t)(x >> shift) & (nb - 1)], 1u);
+    }
+    __syncthreads();
+    block_find_crossing<NT>(hist, (int)nb, rem, scratch, out);
+    const uint32_t b = out[0], above = out[1], cnt = out[2];
+    __syncthreads();
+    if (b == 0xffffffffu) return prefix;  // cannot happen when n >= k
+    prefix |= (KeyT)b << shift;
+    rem -= above;
+    lo = shift;
+    if (cnt == rem) break;  // the whole bin is taken
+  }
+  return prefix;
+}
+
+// In-place bitonic sort of P2 (power of two) 64-bit keys in LDS, descending.
+template <int NT>
+RK_DEV void block_bitonic_desc(uint64_t* a, uint32_t P2) {
+  for (uint32_t size = 2; size <= P2; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (uint32_t i = threadIdx.x; i < (P2 >> 1); i += NT) {
+        const uint32_t lo = 2 * i - (i & (stride - 1));
+        const uint32_t hi = lo + stride;
+        const bool desc = (lo & size) == 0;
+        const uint64_t x = a[lo], y = a[hi];
+        if (desc ? (x < y) : (x > y)) {
+          a[lo] = y;
+          a[hi] = x;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+RK_DEV uint32_t next_pow2(uint32_t x) {
+  return x <= 1 ? 1u : (1u << (32 - __builtin_clz(x - 1)));
+}
+
+}  // namespace rk

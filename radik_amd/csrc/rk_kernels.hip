@@ -1,0 +1,1057 @@
+// gfx950 kernels of the batched radix top-k and their launch sequence.
+//
+// The pipeline replaces radik/RadixSelect/topk_radixselect.h:59-323
+// (topKRadixSelectL) with a fixed launch sequence whose per-row decisions
+// live in device memory (RowState.phase), so the whole call is stream
+// ordered, has no host round trip and can be captured in a hipGraph:
+//
+//   plan_rows, plan_scan            row descriptors, tile list, candidate slots
+//   sample                          long rows: threshold T from a sample
+//   filter                          one streaming read: compact keys >= T
+//   select_finalize                 LDS radix select + bitonic sort + write
+//   exact_init, (hist, hist_select) x passes
+//                                   exact RadiK passes for rows the sampled
+//                                   path could not settle (and the plain path)
+//   filter, select_finalize         second round for those rows
+//   emit_count, emit_scan, emit_write, finalize_ordered
+//                                   rows whose ties at the k-th key overflow
+//                                   the candidate buffer: ordered emit
+//
+// Selection order everywhere: the 64-bit composite key
+//   (sel_key << 32) | ~row_position
+// which is unique per element, so "the k largest composite keys" is the
+// deterministic lowest-position tie-break that radixselect_l.cuh:417-421
+// leaves to an atomic race.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+#include "rk_device.hpp"
+#include "rk_pipeline.hpp"
+
+namespace rk {
+
+constexpr int SEL_NT = 1024;         // threads of the per-row select kernels
+constexpr int LDS_KEYS_MAX = 16384;  // compile-time bound of the LDS key buffer
+constexpr int SEL_ITEMS = LDS_KEYS_MAX / SEL_NT;
+constexpr int STREAM_NT = 256;       // threads of the streaming kernels
+constexpr int STAGE = 2048;          // LDS candidate staging entries per workgroup
+
+// ---------------------------------------------------------------------------
+// Work partition of the streaming kernels: the tiles of all rows form one
+// flat list (tile_start is its per-row prefix).  Workgroup w takes the
+// contiguous range [T*w/G, T*(w+1)/G), so its tiles of one row form one
+// contiguous element range, and the partition is identical in every launch
+// with the same grid (each tile meets the same XCD's L2 in every pass).
+// ---------------------------------------------------------------------------
+template <typename F>
+RK_DEV void for_each_segment(const KParams& p, F&& f) {
+  const uint32_t T = p.tile_start[p.R];
+  const uint64_t G = gridDim.x, w = blockIdx.x;
+  const uint32_t t0 = (uint32_t)((uint64_t)T * w / G);
+  const uint32_t t1 = (uint32_t)((uint64_t)T * (w + 1) / G);
+  if (t0 >= t1) return;
+  int lo = 0, hi = p.R;  // invariant: tile_start[lo] <= t0 < tile_start[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (p.tile_start[mid] <= t0) lo = mid;
+    else hi = mid;
+  }
+  int r = lo;
+  uint32_t t = t0;
+  while (t < t1 && r < p.R) {
+    const uint32_t rs = p.tile_start[r], re = p.tile_start[r + 1];
+    if (re <= t) {
+      ++r;
+      continue;
+    }
+    const uint32_t se = re < t1 ? re : t1;
+    f(r, t - rs, se - rs);
+    t = se;
+    ++r;
+  }
+}
+
+// First element index >= es whose address is 16-byte aligned.
+template <int DT>
+RK_DEV int64_t aligned_start(const typename Traits<DT>::U* rb, int64_t es) {
+  constexpr int ES = sizeof(typename Traits<DT>::U);
+  const uint64_t addr = (uint64_t)(rb + es);
+  return es + (int64_t)(((16u - (addr & 15u)) & 15u) / ES);
+}
+
+// ---------------------------------------------------------------------------
+// plan
+// ---------------------------------------------------------------------------
+template <int DT>
+__global__ __launch_bounds__(256) void k_plan_rows(KParams p) {
+  using U = typename Traits<DT>::U;
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= p.R) return;
+  const U* base;
+  const int32_t* idx = nullptr;
+  int64_t len;
+  if (p.row_ptrs) {
+    base = static_cast<const U*>(p.row_ptrs[r]);
+    len = p.row_len[r];
+  } else {
+    const int64_t off = p.row_off[r];
+    len = p.row_off[r + 1] - off;
+    base = static_cast<const U*>(p.vals) + off;
+    if (p.idx_in) idx = p.idx_in + off;
+  }
+  const int64_t k = p.row_k ? p.row_k[r] : p.K;
+  const int64_t out_pos = p.out_off ? p.out_off[r] : (int64_t)r * p.K;
+  const int64_t slots = p.out_off ? p.out_off[r + 1] - p.out_off[r] : p.K;
+  const int64_t m = len < k ? len : k;
+  RowDesc d;
+  d.base = base;
+  d.idx_in = idx;
+  d.len = len;
+  d.out_pos = out_pos;
+  d.k = (int32_t)k;
+  d.slots = (int32_t)slots;
+  RowState s;
+  s.flags = 0;
+  s.T = s.prefix = s.resolved = s.rem = s.V = 0;
+  s.cap = 0;
+  uint32_t tiles = 0;
+  const bool bad = len < 0 || k < 0 || k > p.max_k || m > slots || slots > 0x7fffffff ||
+                   len > 0x7fffffffLL;
+  if (bad) {
+    s.phase = PH_DONE;
+    s.flags = 0x80000000u;  // reported by plan_scan
+  } else if (len <= k) {
+    s.phase = PH_COPYALL;
+  } else if (len <= p.small_max) {
+    s.phase = PH_SMALL;
+  } else {
+    s.phase = (p.flags & FL_EXACT) ? PH_HIST : PH_SAMPLE;
+    if (s.phase == PH_HIST) s.flags = RF_NEED_INIT;
+    tiles = (uint32_t)((len + p.tune.tile - 1) / p.tune.tile);
+    const int64_t cap = 4 * k + (len + 63) / 64 + 1024;
+    s.cap = (uint32_t)(cap < len ? cap : len);
+  }
+  p.desc[r] = d;
+  p.st[r] = s;
+  p.tile_start[r] = tiles;
+  p.cand_base[r] = s.cap;
+  p.cand_cnt[r] = 0;
+}
+
+// Exclusive scans of the per-row tile counts and candidate capacities (one
+// workgroup; rows are processed in chunks of 1024 with a running carry).
+__global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
+  __shared__ uint32_t scratch[16];
+  __shared__ uint64_t scratch64[16];
+  __shared__ uint32_t bad_sh;
+  __shared__ uint32_t status_sh;
+  const int tid = threadIdx.x;
+  if (tid == 0) bad_sh = 0;
+  __syncthreads();
+  uint32_t carry_t = 0;
+  uint64_t carry_c = 0;
+  for (int base = 0; base < p.R; base += 1024) {
+    const int r = base + tid;
+    const uint32_t t = r < p.R ? p.tile_start[r] : 0u;
+    const uint64_t c = r < p.R ? p.cand_base[r] : 0ull;
+    if (r < p.R && (p.st[r].flags & 0x80000000u)) atomicOr(&bad_sh, 1u);
+    uint32_t tot_t;
+    const uint32_t ex_t = block_excl_scan<1024>(t, scratch, tot_t);
+    // 64-bit scan of the capacities: wave scan + per-wave totals.
+    uint64_t inc = c;
+    const int lane = lane_id(), wid = tid >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint64_t y = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += y;
+    }
+    if (lane == 63) scratch64[wid] = inc;
+    __syncthreads();
+    uint64_t pre = 0, tot_c = 0;
+    for (int w = 0; w < 16; ++w) {
+      const uint64_t v = scratch64[w];
+      pre += (w < wid) ? v : 0ull;
+      tot_c += v;
+    }
+    __syncthreads();
+    if (r < p.R) {
+      p.tile_start[r] = carry_t + ex_t;
+      p.cand_base[r] = carry_c + pre + inc - c;
+    }
+    carry_t += tot_t;
+    carry_c += tot_c;
+  }
+  if (tid == 0) {
+    p.tile_start[p.R] = carry_t;
+    p.cand_base[p.R] = carry_c;
+    uint32_t status = bad_sh ? ST_BAD_ROW : 0u;
+    Header* h = p.hdr;
+    h->total_tiles = carry_t;
+    h->total_cand = carry_c;
+    h->fallback_rows = 0;
+    h->ordered_rows = 0;
+    if (carry_t > p.tile_capacity) status |= ST_TILES;
+    if (carry_c > p.cand_capacity) status |= ST_CANDIDATES;
+    h->status = status;
+    status_sh = status;
+    if (status & (ST_TILES | ST_CANDIDATES)) p.tile_start[p.R] = 0;  // disable streaming
+  }
+  __syncthreads();
+  // On a capacity overflow no long row is processed (the status reports it).
+  if (status_sh & (ST_TILES | ST_CANDIDATES)) {
+    for (int r = tid; r < p.R; r += 1024) {
+      const uint32_t ph = p.st[r].phase;
+      if (ph == PH_SAMPLE || ph == PH_HIST) p.st[r].phase = PH_DONE;
+      p.tile_start[r] = 0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// sample: threshold T for a long row from sample_groups evenly spaced runs of
+// group_elems contiguous elements.  With s sampled keys and mu = s*k/len, T is
+// the r-th largest sampled key, r = ceil(mu + 4 sqrt(mu) + 2); #(row >= T) >=
+// k then holds unless the sample is unrepresentative, which the select kernel
+// detects exactly (it counts) and sends to the exact passes.
+// ---------------------------------------------------------------------------
+template <int DT, bool LARGEST>
+__global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
+  using U = typename Traits<DT>::U;
+  extern __shared__ uint64_t dyn64[];
+  uint32_t* keys = reinterpret_cast<uint32_t*>(dyn64);
+  __shared__ uint32_t hist[HIST_MAX_BINS];
+  __shared__ uint32_t scratch[16];
+  __shared__ uint64_t scratch64[16];
+  __shared__ uint32_t outv[4];
+  for (int r = blockIdx.x; r < p.R; r += gridDim.x) {
+    if (p.st[r].phase != PH_SAMPLE) continue;
+    const RowDesc d = p.desc[r];
+    const U* rb = static_cast<const U*>(d.base);
+    const int64_t len = d.len;
+    const int64_t ge = p.tune.group_elems;
+    int64_t ng = p.tune.sample_groups;
+    const int64_t max_keys = (int64_t)p.lds_keys * 2;
+    if (ng * ge > max_keys) ng = max_keys / ge;
+    if (ng * ge * 64 > len) ng = len / (64 * ge);  // read at most 1/64 of the row
+    if (ng < 1) ng = 1;
+    const uint32_t s = (uint32_t)(ng * ge);
+    for (uint32_t i = threadIdx.x; i < s; i += SEL_NT) {
+      const int64_t g = i / ge, j = i % ge;
+      const int64_t start = ng > 1 ? (g * (len - ge)) / (ng - 1) : 0;
+      keys[i] = sel_key<DT, LARGEST>((uint32_t)rb[start + j]);
+    }
+    __syncthreads();
+    const float mu = (float)s * (float)d.k / (float)len;
+    float rf = ceilf(mu + 4.0f * sqrtf(mu) + 2.0f);
+    uint32_t rank = rf >= (float)s ? s : (uint32_t)rf;
+    if (rank < 1) rank = 1;
+    uint32_t T = 0;
+    if (rank < s) T = block_radix_select<SEL_NT, 11, uint32_t>(keys, s, rank, hist, scratch, scratch64, outv);
+    uint32_t ge_cnt = 0;
+    for (uint32_t i = threadIdx.x; i < s; i += SEL_NT) ge_cnt += (keys[i] >= T) ? 1u : 0u;
+    ge_cnt = block_sum<SEL_NT>(ge_cnt, scratch);
+    if (threadIdx.x == 0) {
+      RowState& st = p.st[r];
+      // Expected candidates: the sampled fraction at or above T, scaled.
+      const double est = (double)ge_cnt * (double)len / (double)s;
+      if (T == 0 || rank >= s || est * 1.25 + 1024.0 > (double)st.cap) {
+        st.phase = PH_HIST;
+        st.flags = RF_NEED_INIT;
+        atomicAdd(&p.hdr->fallback_rows, 1u);
+      } else {
+        st.T = T;
+        st.phase = PH_FILTER;
+        st.flags = 0;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// filter: the streaming pass.  Every element of every PH_FILTER row is read
+// once with 16-byte loads; keys >= T are appended (as composite keys) to the
+// row's candidate slots, staged in LDS and flushed with one global atomic
+// per segment.  The compare is done on the float value (one VALU op per
+// element); it admits every key >= T plus possibly -0/+0 neighbours.
+// ---------------------------------------------------------------------------
+template <int DT, bool LARGEST, int U>
+__global__ __launch_bounds__(STREAM_NT) void k_filter(KParams p) {
+  using UT = typename Traits<DT>::U;
+  constexpr int P = Traits<DT>::PACK;
+  __shared__ uint64_t stage[STAGE];
+  __shared__ uint32_t stage_cnt;
+  __shared__ uint32_t gbase_sh;
+  if (threadIdx.x == 0) stage_cnt = 0;
+  __syncthreads();
+  const int lane = lane_id();
+  const int wave_base = threadIdx.x & ~63;
+
+  for_each_segment(p, [&](int r, uint32_t ta, uint32_t tb) {
+    if (p.st[r].phase != PH_FILTER) return;  // uniform across the block
+    const RowDesc d = p.desc[r];
+    const uint32_t T = p.st[r].T;
+    const uint32_t cap = p.st[r].cap;
+    const uint64_t cbase = p.cand_base[r];
+    const bool all = (T == 0);
+    const float tf = all ? 0.0f : key_threshold_float<DT, LARGEST>(T);
+    const UT* rb = static_cast<const UT*>(d.base);
+    const int64_t es = (int64_t)ta * p.tune.tile;
+    int64_t ee = (int64_t)tb * p.tune.tile;
+    if (ee > d.len) ee = d.len;
+    int64_t al = aligned_start<DT>(rb, es);
+    if (al > ee) al = ee;
+    const int64_t nv = (ee - al) / P;
+    const int64_t tail0 = al + nv * P;
+
+    auto pred = [&](uint32_t u) -> bool {
+      if (all) return true;
+      const float x = bits_to_float<DT>(u);
+      return LARGEST ? (x >= tf) : (x <= tf);
+    };
+    // Wave-wide append of the masked elements of one vector per lane.
+    auto append = [&](const uint32_t* e, uint32_t mask, int64_t first) {
+      const uint32_t cnt = __popc(mask);
+      const uint32_t incl = wave_incl_scan(cnt);
+      const uint32_t tot = __shfl(incl, 63, 64);
+      if (tot == 0) return;
+      uint32_t b = 0;
+      if (lane == 0) b = atomicAdd(&stage_cnt, tot);
+      b = __shfl(b, 0, 64);
+      const bool to_stage = b + tot <= (uint32_t)STAGE;
+      uint32_t gb = 0;
+      if (!to_stage) {
+        if (lane == 0) gb = atomicAdd(&p.cand_cnt[r], tot);
+        gb = __shfl(gb, 0, 64);
+      }
+      uint32_t o = incl - cnt;
+#pragma unroll
+      for (int c = 0; c < P; ++c) {
+        if ((mask >> c) & 1u) {
+          const uint32_t pos = (uint32_t)(first + c);
+          const uint64_t key = ((uint64_t)sel_key<DT, LARGEST>(e[c]) << 32) | (uint32_t)(~pos);
+          if (to_stage) stage[b + o] = key;
+          else if (gb + o < cap) p.cand[cbase + gb + o] = key;
+          ++o;
+        }
+      }
+    };
+
+    // Head and tail elements outside the aligned body: wave 0, one per lane.
+    if (wave_base == 0) {
+      uint32_t e[P];
+#pragma unroll
+      for (int c = 0; c < P; ++c) e[c] = 0;
+      int64_t pos = -1;
+      if (lane < P && es + lane < al) pos = es + lane;
+      if (lane >= 32 && lane - 32 < P && tail0 + (lane - 32) < ee) pos = tail0 + (lane - 32);
+      uint32_t mask = 0;
+      if (pos >= 0) {
+        e[0] = (uint32_t)rb[pos];
+        mask = pred(e[0]) ? 1u : 0u;
+      }
+      if (__ballot(mask != 0)) append(e, mask, pos);
+    }
+    // Aligned body.
+    const uint4* vp = reinterpret_cast<const uint4*>(rb + al);
+    for (int64_t jb = wave_base; jb < nv; jb += (int64_t)STREAM_NT * U) {
+      uint4 x[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t j = jb + lane + (int64_t)u * STREAM_NT;
+        x[u] = j < nv ? vp[j] : make_uint4(0u, 0u, 0u, 0u);
+      }
+      uint32_t m[U];
+      uint32_t anym = 0;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t j = jb + lane + (int64_t)u * STREAM_NT;
+        const Vec<DT> v = unpack<DT>(x[u]);
+        uint32_t mm = 0;
+#pragma unroll
+        for (int c = 0; c < P; ++c) mm |= (pred(v.e[c]) ? 1u : 0u) << c;
+        m[u] = j < nv ? mm : 0u;
+        anym |= m[u];
+      }
+      if (__ballot(anym != 0)) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (__ballot(m[u] != 0)) {
+            const Vec<DT> v = unpack<DT>(x[u]);
+            const int64_t j = jb + lane + (int64_t)u * STREAM_NT;
+            append(v.e, m[u], al + j * P);
+          }
+        }
+      }
+    }
+    // Flush the LDS stage with one global atomic.
+    __syncthreads();
+    const uint32_t n = stage_cnt < (uint32_t)STAGE ? stage_cnt : (uint32_t)STAGE;
+    if (threadIdx.x == 0) gbase_sh = n ? atomicAdd(&p.cand_cnt[r], n) : 0u;
+    __syncthreads();
+    const uint32_t gb = gbase_sh;
+    for (uint32_t i = threadIdx.x; i < n; i += STREAM_NT)
+      if (gb + i < cap) p.cand[cbase + gb + i] = stage[i];
+    __syncthreads();
+    if (threadIdx.x == 0) stage_cnt = 0;
+    __syncthreads();
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Output ordering.  The k selected composite keys are re-keyed so that one
+// descending bitonic sort yields the requested order:
+//   position order (unsorted):  (~pos << 32) | key
+//   best first:                 (key << 32) | ~pos
+//   worst first:                (~key << 32) | ~pos    (ties: position ascending)
+// All re-keyed values are non-zero, so 0 pads the sort to a power of two.
+// ---------------------------------------------------------------------------
+enum OrderMode : int { ORD_POS = 0, ORD_BEST = 1, ORD_WORST = 2 };
+
+RK_DEV int order_mode(uint32_t flags) {
+  if (!(flags & FL_SORTED)) return ORD_POS;
+  const bool largest = flags & FL_LARGEST, ascend = flags & FL_ASCEND;
+  return (largest != ascend) ? ORD_BEST : ORD_WORST;
+}
+
+template <int DT>
+RK_DEV uint64_t order_key(int mode, uint32_t sk, uint32_t pos) {
+  constexpr uint32_t KM = Traits<DT>::KEY_MASK;
+  if (mode == ORD_POS) return ((uint64_t)(~pos) << 32) | sk;
+  if (mode == ORD_BEST) return ((uint64_t)sk << 32) | (uint32_t)(~pos);
+  return ((uint64_t)(~sk & KM) << 32) | (uint32_t)(~pos);
+}
+
+template <int DT>
+RK_DEV void order_decode(int mode, uint64_t key, uint32_t& sk, uint32_t& pos) {
+  constexpr uint32_t KM = Traits<DT>::KEY_MASK;
+  if (mode == ORD_POS) {
+    pos = ~(uint32_t)(key >> 32);
+    sk = (uint32_t)key;
+  } else if (mode == ORD_BEST) {
+    sk = (uint32_t)(key >> 32);
+    pos = ~(uint32_t)key;
+  } else {
+    sk = ~(uint32_t)(key >> 32) & KM;
+    pos = ~(uint32_t)key;
+  }
+}
+
+// Sort sbuf[0, m) (re-keyed) and write the row's output plus its tail fill.
+template <int DT, bool LARGEST>
+RK_DEV void sort_and_write(const KParams& p, const RowDesc& d, uint64_t* sbuf, uint32_t m, int mode) {
+  using U = typename Traits<DT>::U;
+  const uint32_t P2 = next_pow2(m);
+  for (uint32_t i = m + threadIdx.x; i < P2; i += SEL_NT) sbuf[i] = 0;
+  __syncthreads();
+  if (P2 > 1) block_bitonic_desc<SEL_NT>(sbuf, P2);
+  const U* rb = static_cast<const U*>(d.base);
+  U* ov = static_cast<U*>(p.out_vals) + d.out_pos;
+  int32_t* oi = p.out_idx + d.out_pos;
+  for (uint32_t i = threadIdx.x; i < m; i += SEL_NT) {
+    uint32_t sk, pos;
+    order_decode<DT>(mode, sbuf[i], sk, pos);
+    const uint32_t bits = sk ? key_to_bits<DT, LARGEST>(sk) : (uint32_t)rb[pos];
+    ov[i] = (U)bits;
+    oi[i] = d.idx_in ? d.idx_in[pos] : (int32_t)pos;
+  }
+  for (int64_t i = (int64_t)m + threadIdx.x; i < d.slots; i += SEL_NT) {
+    ov[i] = (U)0;
+    oi[i] = -1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// select_finalize: per row, one 1024-thread workgroup with a 16384-key LDS
+// buffer.  COPYALL rows are sorted whole; SMALL rows are loaded straight from
+// the input; FILTER rows read their candidates (LDS when they fit, global
+// otherwise), verify that at least k of them are >= T, radix-select the k
+// largest composite keys, sort and write.  A sampled row that fails the
+// verification goes to the exact passes.
+// ---------------------------------------------------------------------------
+template <int DT, bool LARGEST>
+__global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
+  using U = typename Traits<DT>::U;
+  extern __shared__ uint64_t sbuf[];
+  __shared__ uint32_t hist[HIST_MAX_BINS];
+  __shared__ uint32_t scratch[16];
+  __shared__ uint64_t scratch64[16];
+  __shared__ uint32_t outv[4];
+  __shared__ uint32_t slot_cnt;
+  const int mode = order_mode(p.flags);
+  for (int r = blockIdx.x; r < p.R; r += gridDim.x) {
+    const uint32_t ph = p.st[r].phase;
+    if (ph != PH_COPYALL && ph != PH_SMALL && ph != PH_FILTER) continue;
+    const RowDesc d = p.desc[r];
+    const U* rb = static_cast<const U*>(d.base);
+    const int64_t len = d.len;
+    const uint32_t k = (uint32_t)d.k;
+    if (ph == PH_COPYALL) {
+      const uint32_t m = (uint32_t)(len < (int64_t)k ? len : (int64_t)k);
+      for (uint32_t i = threadIdx.x; i < m; i += SEL_NT)
+        sbuf[i] = order_key<DT>(mode, sel_key<DT, LARGEST>((uint32_t)rb[i]), i);
+      __syncthreads();
+      sort_and_write<DT, LARGEST>(p, d, sbuf, m, mode);
+    } else {
+      const uint64_t* keys;
+      uint32_t n;
+      bool in_lds;
+      if (ph == PH_SMALL) {
+        n = (uint32_t)len;
+        for (uint32_t i = threadIdx.x; i < n; i += SEL_NT)
+          sbuf[i] = ((uint64_t)sel_key<DT, LARGEST>((uint32_t)rb[i]) << 32) | (uint32_t)(~i);
+        keys = sbuf;
+        in_lds = true;
+      } else {
+        const RowState st = p.st[r];
+        n = p.cand_cnt[r];
+        const uint64_t* src = p.cand + p.cand_base[r];
+        bool ok = n <= st.cap && n >= k;
+        if (ok) {
+          in_lds = n <= (uint32_t)p.lds_keys;
+          if (in_lds) {
+            for (uint32_t i = threadIdx.x; i < n; i += SEL_NT) sbuf[i] = src[i];
+            keys = sbuf;
+          } else {
+            keys = src;
+          }
+          __syncthreads();
+          uint32_t ge = 0;
+          for (uint32_t i = threadIdx.x; i < n; i += SEL_NT) ge += ((uint32_t)(keys[i] >> 32) >= st.T) ? 1u : 0u;
+          ge = block_sum<SEL_NT>(ge, scratch);
+          ok = ge >= k;
+        }
+        if (!ok) {
+          if (threadIdx.x == 0) {
+            if (st.flags & RF_EXACT_T) atomicOr(&p.hdr->status, (uint32_t)ST_INTERNAL);
+            p.st[r].phase = PH_HIST;
+            p.st[r].flags = RF_NEED_INIT;
+            atomicAdd(&p.hdr->fallback_rows, 1u);
+          }
+          __syncthreads();
+          continue;
+        }
+      }
+      __syncthreads();
+      const uint64_t kth = block_radix_select<SEL_NT, 11, uint64_t>(keys, n, k, hist, scratch, scratch64, outv);
+      if (threadIdx.x == 0) slot_cnt = 0;
+      if (in_lds) {
+        uint64_t reg[SEL_ITEMS];
+#pragma unroll
+        for (int j = 0; j < SEL_ITEMS; ++j) {
+          const uint32_t i = threadIdx.x + j * SEL_NT;
+          reg[j] = i < n ? sbuf[i] : 0ull;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < SEL_ITEMS; ++j) {
+          const uint32_t i = threadIdx.x + j * SEL_NT;
+          if (i < n && reg[j] >= kth) {
+            const uint32_t slot = atomicAdd(&slot_cnt, 1u);
+            sbuf[slot] = order_key<DT>(mode, (uint32_t)(reg[j] >> 32), ~(uint32_t)reg[j]);
+          }
+        }
+      } else {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += SEL_NT) {
+          const uint64_t x = keys[i];
+          if (x >= kth) {
+            const uint32_t slot = atomicAdd(&slot_cnt, 1u);
+            if (slot < k) sbuf[slot] = order_key<DT>(mode, (uint32_t)(x >> 32), ~(uint32_t)x);
+          }
+        }
+      }
+      __syncthreads();
+      if (threadIdx.x == 0 && slot_cnt != k) atomicOr(&p.hdr->status, (uint32_t)ST_INTERNAL);
+      sort_and_write<DT, LARGEST>(p, d, sbuf, k, mode);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) p.st[r].phase = PH_DONE;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Exact passes (RadiK's countBin / selectBin, radixselect_l.cuh:14-126,
+// 173-267, on the selection key): per pass, a histogram of the next digit of
+// the keys that share the resolved prefix, then the bin holding the k-th
+// key.  A row leaves as soon as the keys >= the selected bin's lower bound
+// fit its candidate slots (threshold known exactly), or, with every bit
+// resolved, as PH_ORDERED.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_exact_init(KParams p) {
+  for (int r = blockIdx.x; r < p.R; r += gridDim.x) {
+    const RowState s = p.st[r];
+    if (s.phase != PH_HIST || !(s.flags & RF_NEED_INIT)) continue;
+    uint32_t* h = p.ghist + (size_t)r * HIST_MAX_BINS;
+    for (int i = threadIdx.x; i < HIST_MAX_BINS; i += 256) h[i] = 0;
+    if (threadIdx.x == 0) {
+      RowState& w = p.st[r];
+      w.prefix = 0;
+      w.resolved = 0;
+      w.rem = (uint32_t)p.desc[r].k;
+      w.flags = 0;
+      p.cand_cnt[r] = 0;
+    }
+  }
+}
+
+template <int DT, bool LARGEST, int U>
+__global__ __launch_bounds__(STREAM_NT) void k_hist(KParams p) {
+  using UT = typename Traits<DT>::U;
+  constexpr int P = Traits<DT>::PACK;
+  constexpr int KB = Traits<DT>::KEY_BITS;
+  constexpr int NW = STREAM_NT / 64;
+  __shared__ uint32_t sub[NW][HIST_MAX_BINS];
+  for (int i = threadIdx.x; i < NW * HIST_MAX_BINS; i += STREAM_NT) (&sub[0][0])[i] = 0;
+  __syncthreads();
+  const int lane = lane_id();
+  const int wave = threadIdx.x >> 6;
+  const int wave_base = threadIdx.x & ~63;
+  uint32_t* h = sub[wave];
+  const int HB = p.tune.hist_bits;
+
+  for_each_segment(p, [&](int r, uint32_t ta, uint32_t tb) {
+    if (p.st[r].phase != PH_HIST) return;
+    const RowDesc d = p.desc[r];
+    const uint32_t b = p.st[r].resolved;
+    const uint32_t P0 = p.st[r].prefix;
+    const int w = (KB - (int)b) < HB ? (KB - (int)b) : HB;
+    const int shift = KB - (int)b - w;
+    const uint32_t nb = 1u << w;
+    const UT* rb = static_cast<const UT*>(d.base);
+    const int64_t es = (int64_t)ta * p.tune.tile;
+    int64_t ee = (int64_t)tb * p.tune.tile;
+    if (ee > d.len) ee = d.len;
+    int64_t al = aligned_start<DT>(rb, es);
+    if (al > ee) al = ee;
+    const int64_t nv = (ee - al) / P;
+    const int64_t tail0 = al + nv * P;
+
+    auto count = [&](uint32_t u, bool valid) {
+      const uint32_t sk = sel_key<DT, LARGEST>(u);
+      const bool ok = valid && (b == 0 || (uint32_t)((uint64_t)sk >> (KB - b)) == P0);
+      const uint32_t dg = (sk >> shift) & (nb - 1);
+      const uint64_t act = __ballot(ok);
+      if (act) {
+        const int fl = __ffsll((unsigned long long)act) - 1;
+        const uint32_t d0 = __shfl(dg, fl, 64);
+        const uint64_t same = __ballot(ok && dg == d0);
+        if (same == act) {
+          if (lane == fl) atomicAdd(&h[d0], (uint32_t)__popcll(act));
+        } else if (ok) {
+          atomicAdd(&h[dg], 1u);
+        }
+      }
+    };
+
+    if (wave_base == 0) {
+      int64_t pos = -1;
+      if (lane < P && es + lane < al) pos = es + lane;
+      if (lane >= 32 && lane - 32 < P && tail0 + (lane - 32) < ee) pos = tail0 + (lane - 32);
+      const uint32_t u = pos >= 0 ? (uint32_t)rb[pos] : 0u;
+      count(u, pos >= 0);
+    }
+    const uint4* vp = reinterpret_cast<const uint4*>(rb + al);
+    for (int64_t jb = wave_base; jb < nv; jb += (int64_t)STREAM_NT * U) {
+      uint4 x[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t j = jb + lane + (int64_t)u * STREAM_NT;
+        x[u] = j < nv ? vp[j] : make_uint4(0u, 0u, 0u, 0u);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t j = jb + lane + (int64_t)u * STREAM_NT;
+        const Vec<DT> v = unpack<DT>(x[u]);
+#pragma unroll
+        for (int c = 0; c < P; ++c) count(v.e[c], j < nv);
+      }
+    }
+    __syncthreads();
+    uint32_t* g = p.ghist + (size_t)r * HIST_MAX_BINS;
+    for (uint32_t i = threadIdx.x; i < nb; i += STREAM_NT) {
+      uint32_t s = 0;
+#pragma unroll
+      for (int q = 0; q < NW; ++q) {
+        s += sub[q][i];
+        sub[q][i] = 0;
+      }
+      if (s) atomicAdd(&g[i], s);
+    }
+    __syncthreads();
+  });
+}
+
+__global__ __launch_bounds__(256) void k_hist_select(KParams p, int key_bits) {
+  __shared__ uint32_t hist[HIST_MAX_BINS];
+  __shared__ uint32_t scratch[16];
+  __shared__ uint32_t outv[4];
+  const int KB = key_bits;
+  const int HB = p.tune.hist_bits;
+  for (int r = blockIdx.x; r < p.R; r += gridDim.x) {
+    const RowState s = p.st[r];
+    if (s.phase != PH_HIST) continue;
+    const int b = (int)s.resolved;
+    const int w = (KB - b) < HB ? (KB - b) : HB;
+    const uint32_t nb = 1u << w;
+    uint32_t* g = p.ghist + (size_t)r * HIST_MAX_BINS;
+    for (uint32_t i = threadIdx.x; i < nb; i += 256) {
+      hist[i] = g[i];
+      g[i] = 0;
+    }
+    __syncthreads();
+    block_find_crossing<256>(hist, (int)nb, s.rem, scratch, outv);
+    if (threadIdx.x == 0) {
+      RowState& st = p.st[r];
+      const uint32_t bin = outv[0];
+      if (bin == 0xffffffffu) {
+        atomicOr(&p.hdr->status, (uint32_t)ST_INTERNAL);
+        st.phase = PH_DONE;
+      } else {
+        const uint32_t prefix = (s.prefix << w) | bin;
+        const uint32_t resolved = (uint32_t)(b + w);
+        const uint32_t rem = s.rem - outv[1];
+        const uint32_t k = (uint32_t)p.desc[r].k;
+        const uint64_t need = (uint64_t)(k - rem) + outv[2];
+        if (need <= s.cap) {
+          st.T = prefix << (KB - (int)resolved);
+          st.phase = PH_FILTER;
+          st.flags = RF_EXACT_T;
+          p.cand_cnt[r] = 0;
+        } else if ((int)resolved == KB) {
+          st.V = prefix;
+          st.rem = rem;
+          st.phase = PH_ORDERED;
+          atomicAdd(&p.hdr->ordered_rows, 1u);
+        } else {
+          st.prefix = prefix;
+          st.resolved = resolved;
+          st.rem = rem;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Ordered emit for rows whose ties at the k-th key V exceed the candidate
+// slots (e.g. an all-equal row).  Per tile, counts of keys > V and == V;
+// a per-row scan over tiles; then each tile writes, in row-position order,
+// its keys > V and those == V whose rank among equals is below rem.  The
+// output position of a selected element is gt_before + min(rem, eq_before).
+// ---------------------------------------------------------------------------
+template <int DT>
+RK_DEV uint32_t load_slot(const typename Traits<DT>::U* rb, int64_t es, int64_t al, int64_t nv,
+                          int64_t ee, int64_t s, Vec<DT>& v, int64_t& first) {
+  constexpr int P = Traits<DT>::PACK;
+  if (s >= 1 && s <= nv) {
+    first = al + (s - 1) * P;
+    v = unpack<DT>(*reinterpret_cast<const uint4*>(rb + first));
+    return (1u << P) - 1u;
+  }
+  int64_t lo, hi;
+  if (s == 0) {
+    lo = es;
+    hi = al < ee ? al : ee;
+  } else {
+    lo = al + nv * P;
+    hi = ee;
+  }
+  first = lo;
+  uint32_t mask = 0;
+#pragma unroll
+  for (int c = 0; c < P; ++c) {
+    const bool ok = lo + c < hi;
+    v.e[c] = ok ? (uint32_t)rb[lo + c] : 0u;
+    mask |= (ok ? 1u : 0u) << c;
+  }
+  return mask;
+}
+
+template <int DT, bool LARGEST>
+__global__ __launch_bounds__(STREAM_NT) void k_emit_count(KParams p) {
+  using UT = typename Traits<DT>::U;
+  constexpr int P = Traits<DT>::PACK;
+  __shared__ uint32_t scratch[16];
+  for_each_segment(p, [&](int r, uint32_t ta, uint32_t tb) {
+    if (p.st[r].phase != PH_ORDERED) return;
+    const RowDesc d = p.desc[r];
+    const uint32_t V = p.st[r].V;
+    const UT* rb = static_cast<const UT*>(d.base);
+    for (uint32_t t = ta; t < tb; ++t) {
+      const int64_t es = (int64_t)t * p.tune.tile;
+      int64_t ee = es + p.tune.tile;
+      if (ee > d.len) ee = d.len;
+      int64_t al = aligned_start<DT>(rb, es);
+      if (al > ee) al = ee;
+      const int64_t nv = (ee - al) / P;
+      uint32_t gt = 0, eq = 0;
+      for (int64_t s = threadIdx.x; s < nv + 2; s += STREAM_NT) {
+        Vec<DT> v;
+        int64_t first;
+        const uint32_t mask = load_slot<DT>(rb, es, al, nv, ee, s, v, first);
+#pragma unroll
+        for (int c = 0; c < P; ++c) {
+          if ((mask >> c) & 1u) {
+            const uint32_t sk = sel_key<DT, LARGEST>(v.e[c]);
+            gt += sk > V ? 1u : 0u;
+            eq += sk == V ? 1u : 0u;
+          }
+        }
+      }
+      gt = block_sum<STREAM_NT>(gt, scratch);
+      eq = block_sum<STREAM_NT>(eq, scratch);
+      if (threadIdx.x == 0) p.tile_cnt[p.tile_start[r] + t] = make_uint2(gt, eq);
+    }
+  });
+}
+
+__global__ __launch_bounds__(1024) void k_emit_scan(KParams p) {
+  __shared__ uint32_t scratch[16];
+  for (int r = blockIdx.x; r < p.R; r += gridDim.x) {
+    if (p.st[r].phase != PH_ORDERED) continue;
+    const uint32_t t0 = p.tile_start[r], t1 = p.tile_start[r + 1];
+    uint32_t cg = 0, ce = 0;
+    for (uint32_t base = t0; base < t1; base += 1024) {
+      const uint32_t t = base + threadIdx.x;
+      const uint2 c = t < t1 ? p.tile_cnt[t] : make_uint2(0u, 0u);
+      uint32_t tg, te;
+      const uint32_t eg = block_excl_scan<1024>(c.x, scratch, tg);
+      const uint32_t ee = block_excl_scan<1024>(c.y, scratch, te);
+      if (t < t1) p.tile_pre[t] = make_uint2(cg + eg, ce + ee);
+      cg += tg;
+      ce += te;
+    }
+  }
+}
+
+template <int DT, bool LARGEST>
+__global__ __launch_bounds__(STREAM_NT) void k_emit_write(KParams p) {
+  using UT = typename Traits<DT>::U;
+  constexpr int P = Traits<DT>::PACK;
+  __shared__ uint32_t scratch[16];
+  for_each_segment(p, [&](int r, uint32_t ta, uint32_t tb) {
+    if (p.st[r].phase != PH_ORDERED) return;
+    const RowDesc d = p.desc[r];
+    const uint32_t V = p.st[r].V;
+    const uint32_t rem = p.st[r].rem;
+    const UT* rb = static_cast<const UT*>(d.base);
+    UT* ov = static_cast<UT*>(p.out_vals) + d.out_pos;
+    int32_t* oi = p.out_idx + d.out_pos;
+    for (uint32_t t = ta; t < tb; ++t) {
+      const uint2 pre = p.tile_pre[p.tile_start[r] + t];
+      uint32_t run_g = pre.x, run_e = pre.y;
+      const int64_t es = (int64_t)t * p.tune.tile;
+      int64_t ee = es + p.tune.tile;
+      if (ee > d.len) ee = d.len;
+      int64_t al = aligned_start<DT>(rb, es);
+      if (al > ee) al = ee;
+      const int64_t nv = (ee - al) / P;
+      for (int64_t s0 = 0; s0 < nv + 2; s0 += STREAM_NT) {
+        const int64_t s = s0 + threadIdx.x;
+        Vec<DT> v;
+        int64_t first = 0;
+        uint32_t mask = 0;
+        if (s < nv + 2) mask = load_slot<DT>(rb, es, al, nv, ee, s, v, first);
+        uint32_t g = 0, q = 0;
+        uint32_t sk[P];
+#pragma unroll
+        for (int c = 0; c < P; ++c) {
+          sk[c] = ((mask >> c) & 1u) ? sel_key<DT, LARGEST>(v.e[c]) : 0u;
+          const bool ok = (mask >> c) & 1u;
+          g += (ok && sk[c] > V) ? 1u : 0u;
+          q += (ok && sk[c] == V) ? 1u : 0u;
+        }
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan<STREAM_NT>(g | (q << 16), scratch, tot);
+        uint32_t gb = run_g + (ex & 0xffffu);
+        uint32_t qb = run_e + (ex >> 16);
+#pragma unroll
+        for (int c = 0; c < P; ++c) {
+          if (!((mask >> c) & 1u)) continue;
+          const uint32_t pos = (uint32_t)(first + c);
+          if (sk[c] > V) {
+            const uint32_t o = gb + (qb < rem ? qb : rem);
+            ov[o] = (UT)v.e[c];
+            oi[o] = (int32_t)pos;
+            ++gb;
+          } else if (sk[c] == V) {
+            if (qb < rem) {
+              const uint32_t o = gb + qb;
+              ov[o] = (UT)v.e[c];
+              oi[o] = (int32_t)pos;
+            }
+            ++qb;
+          }
+        }
+        run_g += tot & 0xffffu;
+        run_e += tot >> 16;
+      }
+    }
+  });
+}
+
+template <int DT, bool LARGEST>
+__global__ __launch_bounds__(SEL_NT) void k_finalize_ordered(KParams p) {
+  using U = typename Traits<DT>::U;
+  extern __shared__ uint64_t sbuf[];
+  const int mode = order_mode(p.flags);
+  for (int r = blockIdx.x; r < p.R; r += gridDim.x) {
+    if (p.st[r].phase != PH_ORDERED) continue;
+    const RowDesc d = p.desc[r];
+    const uint32_t m = (uint32_t)d.k;
+    const U* ov = static_cast<const U*>(p.out_vals) + d.out_pos;
+    const int32_t* oi = p.out_idx + d.out_pos;
+    for (uint32_t i = threadIdx.x; i < m; i += SEL_NT)
+      sbuf[i] = order_key<DT>(mode, sel_key<DT, LARGEST>((uint32_t)ov[i]), (uint32_t)oi[i]);
+    __syncthreads();
+    sort_and_write<DT, LARGEST>(p, d, sbuf, m, mode);
+    __syncthreads();
+    if (threadIdx.x == 0) p.st[r].phase = PH_DONE;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side: the launch sequence.
+// ---------------------------------------------------------------------------
+struct Launch {
+  hipStream_t stream;
+  int row_grid;   // grid of per-row kernels
+  int grid;       // grid of streaming kernels
+};
+
+template <int DT, bool LARGEST>
+static void set_lds_attrs(size_t dyn) {
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sample<DT, LARGEST>),
+                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_select_finalize<DT, LARGEST>),
+                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_finalize_ordered<DT, LARGEST>),
+                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+}
+
+// Optional timing of the streaming filter kernel (the dominant kernel):
+// a pair of HIP events recorded around its first launch in each call.
+struct TimingState {
+  std::mutex mu;
+  bool on = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+};
+static TimingState& timing() {
+  static TimingState t;
+  return t;
+}
+
+static bool timing_begin(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>& ev) {
+  TimingState& t = timing();
+  std::lock_guard<std::mutex> lk(t.mu);
+  if (!t.on) return false;
+  if (!t.pool.empty()) {
+    ev = t.pool.back();
+    t.pool.pop_back();
+  } else {
+    if (hipEventCreate(&ev.first) != hipSuccess) return false;
+    if (hipEventCreate(&ev.second) != hipSuccess) return false;
+  }
+  return hipEventRecord(ev.first, s) == hipSuccess;
+}
+
+static void timing_end(hipStream_t s, const std::pair<hipEvent_t, hipEvent_t>& ev) {
+  TimingState& t = timing();
+  std::lock_guard<std::mutex> lk(t.mu);
+  (void)hipEventRecord(ev.second, s);
+  t.pending.push_back(ev);
+}
+
+template <int DT, bool LARGEST, int U>
+static hipError_t run_typed_u(const KParams& p, const Launch& L) {
+  constexpr int KB = Traits<DT>::KEY_BITS;
+  const size_t dyn = (size_t)p.lds_keys * sizeof(uint64_t);
+  const int rows_blocks = (p.R + 255) / 256;
+  hipStream_t s = L.stream;
+  k_plan_rows<DT><<<rows_blocks, 256, 0, s>>>(p);
+  k_plan_scan<<<1, 1024, 0, s>>>(p);
+  if (!(p.flags & FL_EXACT)) {
+    k_sample<DT, LARGEST><<<L.row_grid, SEL_NT, dyn, s>>>(p);
+    std::pair<hipEvent_t, hipEvent_t> ev;
+    const bool timed = timing_begin(s, ev);
+    k_filter<DT, LARGEST, U><<<L.grid, STREAM_NT, 0, s>>>(p);
+    if (timed) timing_end(s, ev);
+    k_select_finalize<DT, LARGEST><<<L.row_grid, SEL_NT, dyn, s>>>(p);
+  }
+  k_exact_init<<<L.row_grid, 256, 0, s>>>(p);
+  const int passes = (KB + p.tune.hist_bits - 1) / p.tune.hist_bits;
+  for (int i = 0; i < passes; ++i) {
+    k_hist<DT, LARGEST, U><<<L.grid, STREAM_NT, 0, s>>>(p);
+    k_hist_select<<<L.row_grid, 256, 0, s>>>(p, KB);
+  }
+  k_filter<DT, LARGEST, U><<<L.grid, STREAM_NT, 0, s>>>(p);
+  k_select_finalize<DT, LARGEST><<<L.row_grid, SEL_NT, dyn, s>>>(p);
+  k_emit_count<DT, LARGEST><<<L.grid, STREAM_NT, 0, s>>>(p);
+  k_emit_scan<<<L.row_grid, 1024, 0, s>>>(p);
+  k_emit_write<DT, LARGEST><<<L.grid, STREAM_NT, 0, s>>>(p);
+  k_finalize_ordered<DT, LARGEST><<<L.row_grid, SEL_NT, dyn, s>>>(p);
+  return hipGetLastError();
+}
+
+template <int DT, bool LARGEST>
+static hipError_t run_typed(const KParams& p, const Launch& L) {
+  static std::once_flag once;
+  std::call_once(once, [&] { set_lds_attrs<DT, LARGEST>((size_t)LDS_KEYS_MAX * sizeof(uint64_t)); });
+  switch (p.tune.unroll) {
+    case 2: return run_typed_u<DT, LARGEST, 2>(p, L);
+    case 8: return run_typed_u<DT, LARGEST, 8>(p, L);
+    default: return run_typed_u<DT, LARGEST, 4>(p, L);
+  }
+}
+
+// Entry used by the C-ABI layer.
+hipError_t run_pipeline(int dtype, const KParams& p, hipStream_t stream, int row_grid, int grid) {
+  Launch L{stream, row_grid, grid};
+  const bool largest = p.flags & FL_LARGEST;
+  switch (dtype) {
+    case F32: return largest ? run_typed<F32, true>(p, L) : run_typed<F32, false>(p, L);
+    case F16: return largest ? run_typed<F16, true>(p, L) : run_typed<F16, false>(p, L);
+    case BF16: return largest ? run_typed<BF16, true>(p, L) : run_typed<BF16, false>(p, L);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int lds_keys_max() { return LDS_KEYS_MAX; }
+
+int timing_enable(int on) {
+  TimingState& t = timing();
+  std::lock_guard<std::mutex> lk(t.mu);
+  t.on = on != 0;
+  return 0;
+}
+
+// Waits for the recorded events; returns the summed milliseconds and count.
+int timing_read(double* ms, int32_t* launches) {
+  TimingState& t = timing();
+  std::lock_guard<std::mutex> lk(t.mu);
+  double total = 0.0;
+  int32_t n = 0;
+  for (auto& ev : t.pending) {
+    if (hipEventSynchronize(ev.second) != hipSuccess) return -1;
+    float f = 0.0f;
+    if (hipEventElapsedTime(&f, ev.first, ev.second) != hipSuccess) return -1;
+    total += f;
+    ++n;
+    t.pool.push_back(ev);
+  }
+  t.pending.clear();
+  if (ms) *ms = total;
+  if (launches) *launches = n;
+  return 0;
+}
+
+}  // namespace rk

@@ -1,0 +1,119 @@
+// Shared host/device layout of the top-k pipeline: row descriptors, row
+// states, the workspace header and the kernel parameter block.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+namespace rk {
+
+// Row phases.  Each kernel acts only on rows in its phase, so the launch
+// sequence is fixed and needs no host round trip.
+enum Phase : uint32_t {
+  PH_DONE = 0,     // output written
+  PH_COPYALL = 1,  // len <= k: the whole row is the answer
+  PH_SMALL = 2,    // row fits in LDS: select directly from the input
+  PH_SAMPLE = 3,   // long row: estimate a threshold from a sample
+  PH_FILTER = 4,   // threshold T known: compact candidates >= T, then select
+  PH_HIST = 5,     // exact radix passes over the row (fallback / plain path)
+  PH_ORDERED = 6,  // k-th key V known but its ties overflow the candidate
+                   // buffer: ordered emit in row-position order
+};
+
+enum RowFlag : uint32_t {
+  RF_NEED_INIT = 1u,  // PH_HIST row whose histogram/state must be reset
+  RF_EXACT_T = 2u,    // T came from exact histograms: the select cannot fail
+};
+
+enum Status : uint32_t {
+  ST_BAD_ROW = 1u,       // negative length / k, k above max_k, output too small
+  ST_TILES = 2u,         // tile list exceeds the workspace bound
+  ST_CANDIDATES = 4u,    // candidate capacity exceeds the workspace bound
+  ST_INTERNAL = 8u,      // invariant broken (should never be set)
+};
+
+struct RowDesc {
+  const void* base;       // first element of the row
+  const int32_t* idx_in;  // optional caller indices, parallel to base
+  int64_t len;
+  int64_t out_pos;  // first output slot
+  int32_t k;
+  int32_t slots;  // output slots reserved for the row (>= min(len, k))
+};
+
+struct RowState {
+  uint32_t phase;
+  uint32_t flags;
+  uint32_t T;         // PH_FILTER threshold key
+  uint32_t prefix;    // PH_HIST: resolved high key bits
+  uint32_t resolved;  // PH_HIST: number of resolved bits
+  uint32_t rem;       // PH_HIST / PH_ORDERED: elements still to take at the boundary
+  uint32_t V;         // PH_ORDERED: the k-th key
+  uint32_t cap;       // candidate capacity of the row
+};
+
+struct Header {
+  uint32_t status;
+  uint32_t total_tiles;
+  uint64_t total_cand;
+  uint32_t fallback_rows;  // rows that left the sampled path for exact passes
+  uint32_t ordered_rows;   // rows that needed the ordered emit
+  uint32_t pad[58];
+};
+
+// Tunables chosen by the autotuner per shape bucket.
+struct Tune {
+  int32_t tile;           // elements per tile (multiple of 64)
+  int32_t grid;           // workgroups of the persistent streaming kernels
+  int32_t sample_groups;  // groups of contiguous elements sampled per row
+  int32_t group_elems;    // elements per sample group (multiple of 8)
+  int32_t hist_bits;      // digit width of the exact passes (8..11)
+  int32_t unroll;         // 16-byte loads in flight per lane (2, 4 or 8)
+};
+
+struct KParams {
+  // inputs
+  const void* vals;
+  const int64_t* row_off;
+  const int32_t* row_k;
+  const int64_t* out_off;
+  const int32_t* idx_in;
+  const void* const* row_ptrs;  // pointer form (serving)
+  const int64_t* row_len;       // pointer form (serving)
+  void* out_vals;
+  int32_t* out_idx;
+  int32_t K;      // k for every row when row_k is null; slot stride when out_off is null
+  int32_t R;      // rows
+  int32_t max_k;
+  uint32_t flags;
+  int64_t total_len;  // bound on sum of lengths (tile / candidate sizing)
+  // workspace
+  Header* hdr;
+  RowDesc* desc;
+  RowState* st;
+  uint32_t* tile_start;  // [R + 1]
+  uint64_t* cand_base;   // [R + 1]
+  uint32_t* cand_cnt;    // [R]
+  uint32_t* ghist;       // [R][2048]
+  uint2* tile_cnt;       // [tile_capacity]
+  uint2* tile_pre;       // [tile_capacity]
+  uint64_t* cand;        // [cand_capacity]
+  uint64_t tile_capacity;
+  uint64_t cand_capacity;
+  int32_t lds_keys;  // LDS key capacity of the select kernels
+  int32_t small_max; // rows up to this length select straight from the input
+  Tune tune;
+};
+
+// Public flag bits (mirrors of the C-ABI values).
+enum : uint32_t {
+  FL_LARGEST = 1u,
+  FL_SORTED = 2u,
+  FL_ASCEND = 4u,
+  FL_EXACT = 8u,
+};
+
+constexpr int HIST_MAX_BINS = 2048;
+constexpr int TILE_MIN = 1024;
+
+}  // namespace rk

@@ -1,0 +1,230 @@
+"""GPU parity: the HIP path (through the C-ABI) against the golden vectors and
+the oracle, bit-exact on values and indices, on seeded inputs covering every
+path of the pipeline (copy-all, LDS-direct, sampled filter, exact passes,
+ordered emit), every dtype and flag combination, ragged / misaligned rows,
+ties, NaN / inf / signed zeros, and the reference-surface compat entry."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import DT_CODE, NP_BITS, case_arrays, from_device_bits, gpu_topk, load_golden
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+LARGEST, SORTED, ASCEND, EXACT = 1, 2, 4, 8
+FLAG_SETS = [LARGEST | SORTED, SORTED | ASCEND, LARGEST | SORTED | ASCEND, SORTED, LARGEST, 0]
+
+
+def _oracle(dt, bits, off, ks, flags, idx_in=None):
+    return orc.topk(bits, off, ks, dt=DT_CODE[dt], flags=flags & 7, idx_in=idx_in, nthreads=8)
+
+
+def _check(dt, bits, off, ks, flags, pad_front=0, idx_in=None, expect_clean=True):
+    gb, gi, st = gpu_topk(dt, bits, off, ks, flags, idx_in=idx_in, pad_front=pad_front)
+    ob, oi = _oracle(dt, bits, off, ks, flags, idx_in=idx_in)
+    if not np.array_equal(gi, oi) or not np.array_equal(gb, ob):
+        bad = np.nonzero((gi != oi) | (gb != ob))[0]
+        raise AssertionError(f"{dt} flags={flags} mismatch at {bad[:10]} of {len(gi)}; "
+                             f"gpu idx {gi[bad[:5]]} oracle idx {oi[bad[:5]]} status {st}")
+    return st
+
+
+def _gen_bits(dt, kind, n, seed, a=0.0, b=1.0, dup=0.0, row_len=0):
+    import torch
+
+    import radik_amd as rk
+    tdt = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16}[dt]
+    t = torch.empty(max(n, 1), dtype=tdt, device="cuda")
+    rk.gen_fill(t[:n], kind, seed, 0, a=a, b=b, dup_frac=dup, row_len=row_len)
+    torch.cuda.synchronize()
+    return from_device_bits(t, dt)[:n]
+
+
+@pytest.mark.parametrize("ci", range(72))
+@pytest.mark.parametrize("exact", [False, True])
+def test_golden_vectors(ci, exact):
+    case = load_golden()["cases"][ci]
+    bits, off, ks, oo, exp_bits, exp_idx = case_arrays(case)
+    dt = case["dtype"]
+    gb, gi, _ = gpu_topk(dt, bits, off, ks, case["flags"] | (EXACT if exact else 0), out_off=oo)
+    np.testing.assert_array_equal(gi, exp_idx)
+    np.testing.assert_array_equal(gb, exp_bits)
+
+
+def _ragged(seed, nrows, lo, hi):
+    rng = np.random.default_rng(seed)
+    lens = np.floor(2.0 ** rng.uniform(np.log2(lo), np.log2(hi), nrows)).astype(np.int64)
+    off = np.zeros(nrows + 1, np.int64)
+    off[1:] = np.cumsum(lens)
+    return rng, lens, off
+
+
+@pytest.mark.parametrize("dt", ["f32", "bf16", "f16"])
+@pytest.mark.parametrize("flags", FLAG_SETS)
+@pytest.mark.parametrize("exact", [False, True])
+def test_ragged_normal_rows(dt, flags, exact):
+    rng, lens, off = _ragged(11 + DT_CODE[dt], 24, 300, 400_000)
+    ks = np.minimum(np.floor(2.0 ** rng.uniform(0, 12, len(lens))), 4096).astype(np.int32)
+    bits = _gen_bits(dt, 1, int(off[-1]), 100 + flags)
+    _check(dt, bits, off, ks, flags | (EXACT if exact else 0))
+
+
+@pytest.mark.parametrize("pad", [1, 2, 3])
+def test_misaligned_rows(pad):
+    rng, lens, off = _ragged(21, 12, 17000, 90_000)
+    lens[::3] += 1  # odd lengths shift every later row start
+    off[1:] = np.cumsum(lens)
+    ks = rng.integers(1, 2000, len(lens)).astype(np.int32)
+    bits = _gen_bits("f32", 1, int(off[-1]), 7)
+    for exact in (False, True):
+        _check("f32", bits, off, ks, LARGEST | SORTED | (EXACT if exact else 0), pad_front=pad)
+    b16 = _gen_bits("bf16", 1, int(off[-1]), 8)
+    _check("bf16", b16, off, ks, SORTED | ASCEND, pad_front=pad)
+
+
+@pytest.mark.parametrize("exact", [False, True])
+def test_all_equal_long_rows_take_lowest_positions(exact):
+    """Every key ties: the ties overflow the candidate slots and the ordered
+    emit must return positions 0..k-1 (a row of 2^18 equal values)."""
+    n = 1 << 18
+    off = np.array([0, n, 2 * n, 3 * n], np.int64)
+    bits = np.concatenate([np.full(n, 0x40200000, np.uint32), np.zeros(n, np.uint32),
+                           np.full(n, 0xBF800000, np.uint32)])
+    ks = np.array([1000, 16384, 1], np.int32)
+    for flags in (LARGEST | SORTED, SORTED | ASCEND, LARGEST):
+        st = _check("f32", bits, off, ks, flags | (EXACT if exact else 0))
+        assert st[2] >= 2  # rows 0 and 1 went through the ordered emit
+
+
+@pytest.mark.parametrize("largest", [True, False])
+def test_near_constant_with_duplicates(largest):
+    n = 1 << 19
+    off = np.arange(0, 4 * n + 1, n, dtype=np.int64)
+    bits = _gen_bits("f32", 3, 4 * n, 31, dup=0.3, row_len=n)
+    ks = np.array([16384, 4096, 100, 1], np.int32)
+    for flags in ((LARGEST if largest else ASCEND) | SORTED, LARGEST if largest else 0):
+        for exact in (False, True):
+            _check("f32", bits, off, ks, flags | (EXACT if exact else 0))
+
+
+@pytest.mark.parametrize("largest", [True, False])
+def test_zipf_rows(largest):
+    n = 1 << 19
+    off = np.arange(0, 3 * n + 1, n, dtype=np.int64)
+    bits = _gen_bits("f32", 2, 3 * n, 41, a=1.1, b=2.0 ** 20)
+    ks = np.array([16384, 3000, 7], np.int32)
+    flags = (LARGEST if largest else ASCEND) | SORTED
+    for exact in (False, True):
+        _check("f32", bits, off, ks, flags | (EXACT if exact else 0))
+
+
+@pytest.mark.parametrize("dt", ["f32", "bf16"])
+def test_special_values(dt):
+    rng = np.random.default_rng(5)
+    n = 70_000
+    bits = _gen_bits(dt, 1, 4 * n, 9, a=0.0, b=2.0)
+    specials = {"f32": [0x7FC00000, 0xFFC00000, 0x7F800000, 0xFF800000, 0, 0x80000000],
+                "bf16": [0x7FC0, 0xFFC0, 0x7F80, 0xFF80, 0, 0x8000]}[dt]
+    pos = rng.integers(0, 4 * n, 20000)
+    bits[pos] = np.array(specials, dtype=NP_BITS[dt])[rng.integers(0, 6, 20000)]
+    off = np.arange(0, 4 * n + 1, n, dtype=np.int64)
+    ks = np.array([4096, 1, 300, 16384], np.int32)
+    for flags in FLAG_SETS:
+        for exact in (False, True):
+            _check(dt, bits, off, ks, flags | (EXACT if exact else 0))
+
+
+def test_nan_heavy_row_selects_nan_last():
+    n = 40_000
+    bits = np.full(n, 0x7FC00000, np.uint32)
+    bits[::10] = np.arange(0, n, 10, dtype=np.float32).view(np.uint32)
+    off = np.array([0, n], np.int64)
+    for exact in (False, True):
+        _check("f32", bits, off, np.array([6000], np.int32), LARGEST | SORTED | (EXACT if exact else 0))
+
+
+def test_idx_in_gathered():
+    rng, lens, off = _ragged(51, 8, 100, 200_000)
+    ks = rng.integers(1, 3000, len(lens)).astype(np.int32)
+    bits = _gen_bits("f32", 1, int(off[-1]), 52)
+    idx_in = rng.integers(-2**31, 2**31 - 1, int(off[-1])).astype(np.int32)
+    for flags in (LARGEST | SORTED, SORTED | ASCEND, 0):
+        _check("f32", bits, off, ks, flags, idx_in=idx_in)
+
+
+def test_k_edges_and_empty_rows():
+    lens = np.array([0, 1, 5, 16384, 16385, 100_000, 1 << 20, 3], np.int64)
+    ks = np.array([5, 1, 0, 16384, 16384, 1, 16384, 3], np.int32)
+    off = np.zeros(len(lens) + 1, np.int64)
+    off[1:] = np.cumsum(lens)
+    bits = _gen_bits("f32", 1, int(off[-1]), 61)
+    for flags in (LARGEST | SORTED, SORTED | ASCEND):
+        for exact in (False, True):
+            _check("f32", bits, off, ks, flags | (EXACT if exact else 0))
+
+
+def test_compat_surface_tail_fill():
+    """topk_radix_select mirrors topKRadixSelectL: [taskNum][K] output, one K,
+    rows shorter than K tail-filled with index -1 / value 0."""
+    import torch
+
+    import radik_amd as rk
+    task_len = [5000, 70000, 12, 0, 40000, 300000]
+    off = np.zeros(len(task_len) + 1, np.int64)
+    off[1:] = np.cumsum(task_len)
+    K = 2048
+    bits = _gen_bits("f32", 1, int(off[-1]), 71)
+    vals = torch.from_numpy(bits.view(np.int32).copy()).cuda().view(torch.float32)
+    for largest, ascend in ((False, True), (True, False), (True, True)):
+        vo, io = rk.topk_radix_select(vals, K, task_len, largest=largest, ascend=ascend)
+        torch.cuda.synchronize()
+        flags = SORTED | (LARGEST if largest else 0) | (ASCEND if ascend else 0)
+        ob, oi = orc.topk(bits, off, None, K=K, flags=flags)
+        np.testing.assert_array_equal(io.cpu().numpy().ravel(), oi)
+        np.testing.assert_array_equal(vo.cpu().numpy().view(np.uint32).ravel(), ob)
+
+
+def test_workload_c2_full_size():
+    """Config 2 at its full size (512 x 2^17 N(0,1), k = 1024) vs the oracle."""
+    import radik_amd.workloads as W
+    w = W.c2()
+    vals = W.materialize(w)
+    bits = from_device_bits(vals, "f32")
+    off = w.offsets()
+    for exact in (False, True):
+        _check("f32", bits, off, w.ks, LARGEST | SORTED | (EXACT if exact else 0))
+
+
+@pytest.mark.parametrize("dt", ["f32", "bf16"])
+def test_workload_c4_scaled(dt):
+    """Config 4's shapes (2048 uneven rows, per-row k) with lengths / 16."""
+    import radik_amd.workloads as W
+    w = W.c4(dt, scale=16)
+    vals = W.materialize(w)
+    bits = from_device_bits(vals, dt)
+    st = _check(dt, bits, w.offsets(), w.ks, LARGEST | SORTED)
+    assert st[0] == 0
+
+
+def test_workload_c3_scaled():
+    import radik_amd.workloads as W
+    w = W.c3(scale=8, rows=8)
+    vals = W.materialize(w)
+    bits = from_device_bits(vals, "f32")
+    for flags in (LARGEST | SORTED, SORTED | ASCEND):
+        for exact in (False, True):
+            _check("f32", bits, w.offsets(), w.ks, flags | (EXACT if exact else 0))
+
+
+def test_repeat_calls_reuse_workspace_deterministic():
+    """Same inputs, same workspace, three calls: identical outputs."""
+    rng, lens, off = _ragged(81, 40, 1000, 500_000)
+    ks = rng.integers(1, 4096, len(lens)).astype(np.int32)
+    bits = _gen_bits("f32", 1, int(off[-1]), 82)
+    outs = [gpu_topk("f32", bits, off, ks, LARGEST | SORTED)[:2] for _ in range(3)]
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o[0], outs[0][0])
+        np.testing.assert_array_equal(o[1], outs[0][1])
