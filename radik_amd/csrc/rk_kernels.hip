@@ -552,7 +552,7 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
           const uint32_t i = threadIdx.x + j * SEL_NT;
           if (i < n && reg[j] >= kth) {
             const uint32_t slot = atomicAdd(&slot_cnt, 1u);
-            sbuf[slot] = order_key<DT>(mode, (uint32_t)(reg[j] >> 32), ~(uint32_t)reg[j]);
+            if (slot < k) sbuf[slot] = order_key<DT>(mode, (uint32_t)(reg[j] >> 32), ~(uint32_t)reg[j]);
           }
         }
       } else {
@@ -841,6 +841,7 @@ __global__ __launch_bounds__(STREAM_NT) void k_emit_write(KParams p) {
     const RowDesc d = p.desc[r];
     const uint32_t V = p.st[r].V;
     const uint32_t rem = p.st[r].rem;
+    const uint32_t kk = (uint32_t)d.k;
     const UT* rb = static_cast<const UT*>(d.base);
     UT* ov = static_cast<UT*>(p.out_vals) + d.out_pos;
     int32_t* oi = p.out_idx + d.out_pos;
@@ -878,14 +879,18 @@ __global__ __launch_bounds__(STREAM_NT) void k_emit_write(KParams p) {
           const uint32_t pos = (uint32_t)(first + c);
           if (sk[c] > V) {
             const uint32_t o = gb + (qb < rem ? qb : rem);
-            ov[o] = (UT)v.e[c];
-            oi[o] = (int32_t)pos;
+            if (o < kk) {
+              ov[o] = (UT)v.e[c];
+              oi[o] = (int32_t)pos;
+            }
             ++gb;
           } else if (sk[c] == V) {
             if (qb < rem) {
               const uint32_t o = gb + qb;
-              ov[o] = (UT)v.e[c];
-              oi[o] = (int32_t)pos;
+              if (o < kk) {
+                ov[o] = (UT)v.e[c];
+                oi[o] = (int32_t)pos;
+              }
             }
             ++qb;
           }
