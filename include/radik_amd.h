@@ -80,14 +80,16 @@ int rk_topk_batched(int dtype, const void* vals, const int64_t* row_off,
                     void* out_vals, int32_t* out_idx, const int64_t* out_off,
                     void* workspace, size_t ws_bytes, void* stream);
 
-/* Same, with each row at its own device address (row_ptrs, row_len and row_k
- * are device arrays): the form a serving loop uses for resident requests.
+/* Same, with each row and its output at their own device addresses
+ * (row_ptrs, row_len, row_k, out_val_ptrs and out_idx_ptrs are device
+ * arrays; row r writes row_k[r] slots at out_val_ptrs[r] / out_idx_ptrs[r]):
+ * the form a serving loop uses for requests resident in their own buffers.
  * No counterpart in the reference (it takes one contiguous valIn). */
 int rk_topk_rows(int dtype, const void* const* row_ptrs, const int64_t* row_len,
                  const int32_t* row_k, int32_t num_rows, int64_t total_len,
-                 int32_t max_k, uint32_t flags, void* out_vals,
-                 int32_t* out_idx, const int64_t* out_off, void* workspace,
-                 size_t ws_bytes, void* stream);
+                 int32_t max_k, uint32_t flags, void* const* out_val_ptrs,
+                 int32_t* const* out_idx_ptrs, void* workspace, size_t ws_bytes,
+                 void* stream);
 
 /* Workspace bytes for rk_topk_compat: same arguments as
  * getRadixSelectLWorkSpaceSize<ValType>(K, n, taskNum, &bytes)
@@ -137,6 +139,32 @@ int rk_set_tune(const rk_tune* in);
  * summed kernel milliseconds and the number of launches timed, and clears. */
 int rk_timing_enable(int on);
 int rk_timing_read(double* filter_ms, int32_t* launches);
+
+/* Autotuner (no counterpart in the reference, whose grid constants are fixed
+ * for one GPU, topk_radixselect.h:17): times the batched call on the given
+ * buffers over a grid of tunables and caches the fastest for the shape bucket
+ * (dtype, RK_EXACT, ceil log2 of total_len, num_rows and max_k); later batched
+ * calls in that bucket use it.  Returns the winner and its median time. */
+int rk_autotune_batched(int dtype, const void* vals, const int64_t* row_off,
+                        const int32_t* row_k, int32_t num_rows, int64_t total_len,
+                        int32_t max_k, uint32_t flags, void* out_vals,
+                        int32_t* out_idx, const int64_t* out_off, void* workspace,
+                        size_t ws_bytes, void* stream, int32_t reps,
+                        rk_tune* best, float* best_ms);
+int rk_tune_cache_save(const char* path);
+int rk_tune_cache_load(const char* path);
+int rk_tune_cache_clear(void);
+int rk_tune_cache_size(void);
+
+/* hipGraph capture of any sequence of radik_amd calls issued on `stream`
+ * between rk_graph_begin and rk_graph_end (the batched entry points make no
+ * host round trip and no memset, so they replay correctly); replaying reads
+ * the buffers' current contents. */
+typedef struct rk_graph rk_graph;
+int rk_graph_begin(void* stream);
+int rk_graph_end(void* stream, rk_graph** out);
+int rk_graph_launch(rk_graph* graph, void* stream);
+int rk_graph_destroy(rk_graph* graph);
 
 #ifdef __cplusplus
 }

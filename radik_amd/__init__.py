@@ -25,7 +25,8 @@ __all__ = [
     "RK_F32", "RK_F16", "RK_BF16", "RK_LARGEST", "RK_SORTED", "RK_ASCEND", "RK_EXACT",
     "RadikError", "lib", "lib_path", "max_k", "get_workspace_size", "workspace_size",
     "topk_radix_select", "topk_batched", "topk_rows", "status", "gen_fill",
-    "get_tune", "set_tune", "timing_enable", "timing_read",
+    "get_tune", "set_tune", "timing_enable", "timing_read", "autotune_batched",
+    "tune_cache_save", "tune_cache_load", "tune_cache_clear", "Graph",
 ]
 
 RK_F32, RK_F16, RK_BF16 = 0, 1, 2
@@ -47,6 +48,9 @@ EXPORTED = (
     "rk_max_k", "rk_topk_workspace_size", "rk_topk_batched", "rk_topk_rows",
     "rk_compat_workspace_size", "rk_topk_compat", "rk_topk_status", "rk_strerror",
     "rk_get_tune", "rk_set_tune", "rk_gen_fill", "rk_timing_enable", "rk_timing_read",
+    "rk_autotune_batched", "rk_tune_cache_save", "rk_tune_cache_load", "rk_tune_cache_clear",
+    "rk_tune_cache_size", "rk_graph_begin", "rk_graph_end", "rk_graph_launch",
+    "rk_graph_destroy",
 )
 
 
@@ -82,7 +86,7 @@ def lib():
     L.rk_topk_workspace_size.argtypes = [c.c_int, i32, i64, i32, u32, c.POINTER(sz)]
     L.rk_topk_batched.argtypes = [c.c_int, vp, vp, vp, i32, i64, i32, u32, vp, vp, vp, vp,
                                   vp, sz, vp]
-    L.rk_topk_rows.argtypes = [c.c_int, vp, vp, vp, i32, i64, i32, u32, vp, vp, vp, vp, sz, vp]
+    L.rk_topk_rows.argtypes = [c.c_int, vp, vp, vp, i32, i64, i32, u32, vp, vp, vp, sz, vp]
     L.rk_compat_workspace_size.argtypes = [c.c_int, i32, i32, i32, u32, c.POINTER(sz)]
     L.rk_topk_compat.argtypes = [c.c_int, vp, vp, vp, vp, vp, sz, vp, i32, i32, u32, vp]
     L.rk_topk_status.argtypes = [vp, vp, c.POINTER(u32), c.POINTER(u32), c.POINTER(u32)]
@@ -92,6 +96,18 @@ def lib():
     L.rk_set_tune.argtypes = [c.POINTER(_Tune)]
     L.rk_gen_fill.argtypes = [c.c_int, c.c_int, c.c_uint64, c.c_uint64, i64, i64, c.c_double,
                               c.c_double, c.c_double, vp, vp]
+    L.rk_autotune_batched.argtypes = [c.c_int, vp, vp, vp, i32, i64, i32, u32, vp, vp, vp, vp,
+                                      sz, vp, i32, c.POINTER(_Tune), c.POINTER(c.c_float)]
+    L.rk_tune_cache_save.argtypes = [c.c_char_p]
+    L.rk_tune_cache_load.argtypes = [c.c_char_p]
+    L.rk_graph_begin.argtypes = [vp]
+    L.rk_graph_end.argtypes = [vp, c.POINTER(vp)]
+    L.rk_graph_launch.argtypes = [vp, vp]
+    L.rk_graph_destroy.argtypes = [vp]
+    for name in ("rk_autotune_batched", "rk_tune_cache_save", "rk_tune_cache_load",
+                 "rk_tune_cache_clear", "rk_tune_cache_size", "rk_graph_begin", "rk_graph_end",
+                 "rk_graph_launch", "rk_graph_destroy"):
+        getattr(L, name).restype = c.c_int
     L.rk_timing_enable.argtypes = [c.c_int]
     L.rk_timing_read.argtypes = [c.POINTER(c.c_double), c.POINTER(i32)]
     for name in ("rk_timing_enable", "rk_timing_read",
@@ -233,15 +249,15 @@ def topk_batched(values, row_off, row_k, *, largest: bool = True, sorted: bool =
     return out_vals, out_idx, out_off, workspace
 
 
-def topk_rows(row_ptrs, row_len, row_k, out_off, out_vals, out_idx, *, dtype: int, total_len: int,
-              max_k_: int, largest: bool = True, sorted: bool = True, ascend: bool | None = None,
-              exact: bool = False, workspace, stream=None) -> None:
-    """Pointer form: rows at arbitrary device addresses (device int64 pointer array)."""
+def topk_rows(row_ptrs, row_len, row_k, out_val_ptrs, out_idx_ptrs, *, dtype: int,
+              total_len: int, max_k_: int, largest: bool = True, sorted: bool = True,
+              ascend: bool | None = None, exact: bool = False, workspace, stream=None) -> None:
+    """Pointer form: each row and its k output slots at their own device
+    addresses (device int64 tensors of pointers)."""
     flags = _flags(largest, sorted, ascend, exact)
     rc = lib().rk_topk_rows(dtype, _ptr(row_ptrs), _ptr(row_len), _ptr(row_k), row_k.numel(),
-                            total_len, max_k_, flags, _ptr(out_vals), _ptr(out_idx),
-                            _ptr(out_off), _ptr(workspace), workspace.numel(),
-                            _stream_ptr(stream))
+                            total_len, max_k_, flags, _ptr(out_val_ptrs), _ptr(out_idx_ptrs),
+                            _ptr(workspace), workspace.numel(), _stream_ptr(stream))
     _check(rc, "rk_topk_rows")
 
 
@@ -272,6 +288,83 @@ def timing_read():
     ms, n = ctypes.c_double(0.0), ctypes.c_int32(0)
     _check(lib().rk_timing_read(ctypes.byref(ms), ctypes.byref(n)), "rk_timing_read")
     return float(ms.value), int(n.value)
+
+
+def autotune_batched(values, row_off, row_k, out_vals, out_idx, out_off, workspace, *,
+                     total_len: int, max_k_: int, largest: bool = True, sorted: bool = True,
+                     ascend: bool | None = None, exact: bool = False, reps: int = 3, stream=None):
+    """Runs the autotuner on these buffers; returns (best tunables, median ms)."""
+    t = _Tune()
+    ms = ctypes.c_float(0.0)
+    flags = _flags(largest, sorted, ascend, exact)
+    rc = lib().rk_autotune_batched(_dtype_code(values), _ptr(values), _ptr(row_off), _ptr(row_k),
+                                   row_k.numel(), total_len, max_k_, flags, _ptr(out_vals),
+                                   _ptr(out_idx), _ptr(out_off), _ptr(workspace),
+                                   workspace.numel(), _stream_ptr(stream), reps, ctypes.byref(t),
+                                   ctypes.byref(ms))
+    _check(rc, "rk_autotune_batched")
+    return {n: getattr(t, n) for n, _ in _Tune._fields_}, float(ms.value)
+
+
+def tune_cache_save(path: str) -> None:
+    _check(lib().rk_tune_cache_save(path.encode()), "rk_tune_cache_save")
+
+
+def tune_cache_load(path: str) -> None:
+    _check(lib().rk_tune_cache_load(path.encode()), "rk_tune_cache_load")
+
+
+def tune_cache_clear() -> None:
+    _check(lib().rk_tune_cache_clear(), "rk_tune_cache_clear")
+
+
+class Graph:
+    """A hipGraph of radik_amd calls captured on a (non-null) stream:
+
+        with rk.Graph.capture(stream) as g:
+            rk.topk_batched(..., stream=stream)
+        g.launch(stream)
+    """
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def capture(cls, stream):
+        return _GraphCapture(stream)
+
+    def launch(self, stream=None) -> None:
+        _check(lib().rk_graph_launch(self._h, _stream_ptr(stream)), "rk_graph_launch")
+
+    def close(self) -> None:
+        if self._h:
+            lib().rk_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _GraphCapture:
+    def __init__(self, stream):
+        self.stream = stream
+        self.graph = None
+
+    def __enter__(self):
+        _check(lib().rk_graph_begin(_stream_ptr(self.stream)), "rk_graph_begin")
+        self.graph = Graph(None)
+        return self.graph
+
+    def __exit__(self, et, ev, tb):
+        h = ctypes.c_void_p()
+        rc = lib().rk_graph_end(_stream_ptr(self.stream), ctypes.byref(h))
+        if et is None:
+            _check(rc, "rk_graph_end")
+            self.graph._h = h.value
+        return False
 
 
 def get_tune() -> dict:

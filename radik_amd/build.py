@@ -23,6 +23,7 @@ SOURCES = [
     "rk_kernels.hip",
     "rk_api.cpp",
     "rk_gen.hip",
+    "rk_tune.cpp",
 ]
 
 COMMON = [

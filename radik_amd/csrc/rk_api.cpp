@@ -15,6 +15,7 @@ hipError_t run_pipeline(int dtype, const KParams& p, hipStream_t stream, int row
 int lds_keys_max();
 int timing_enable(int on);
 int timing_read(double* ms, int32_t* launches);
+bool tune_lookup(int dtype, uint32_t flags, int32_t rows, int64_t total, int32_t max_k, Tune* t);
 }  // namespace rk
 
 namespace {
@@ -91,9 +92,10 @@ int elem_size(int dtype) { return dtype == RK_F32 ? 4 : 2; }
 
 bool valid_dtype(int dtype) { return dtype == RK_F32 || dtype == RK_F16 || dtype == RK_BF16; }
 
-Tune current_tune(const DevInfo& di) {
+Tune current_tune(const DevInfo& di, int dtype, uint32_t flags, int32_t rows, int64_t total,
+                  int32_t max_k) {
   Tune t;
-  {
+  if (!rk::tune_lookup(dtype, flags, rows, total, max_k, &t)) {
     std::lock_guard<std::mutex> lk(g_mu);
     t = g_tune;
   }
@@ -109,15 +111,23 @@ Tune current_tune(const DevInfo& di) {
 }
 
 // Shared body of the batched entry points.
+struct Outputs {
+  void* vals = nullptr;
+  int32_t* idx = nullptr;
+  const int64_t* off = nullptr;
+  void* const* val_ptrs = nullptr;
+  int32_t* const* idx_ptrs = nullptr;
+};
+
 int launch(int dtype, const void* vals, const int64_t* row_off, const void* const* row_ptrs,
            const int64_t* row_len, const int32_t* row_k, int32_t K, int32_t num_rows,
            int64_t total_len, int32_t max_k, uint32_t flags, const int32_t* idx_in,
-           void* out_vals, int32_t* out_idx, const int64_t* out_off, void* workspace,
-           size_t ws_bytes, void* stream) {
+           const Outputs& out, void* workspace, size_t ws_bytes, void* stream) {
   if (!valid_dtype(dtype)) return RK_E_UNSUPPORTED;
   if (num_rows < 0 || total_len < 0 || max_k < 0) return RK_E_ARG;
   if (num_rows == 0) return RK_OK;
-  if (!out_vals || !out_idx) return RK_E_ARG;
+  if (out.val_ptrs ? !out.idx_ptrs : (!out.vals || !out.idx)) return RK_E_ARG;
+  void* out_vals = out.vals;
   if (row_ptrs ? !row_len : (!vals || !row_off)) return RK_E_ARG;
   const DevInfo di = device_info();
   if (!di.ok) return RK_E_HIP;
@@ -134,12 +144,14 @@ int launch(int dtype, const void* vals, const int64_t* row_off, const void* cons
   p.vals = vals;
   p.row_off = row_off;
   p.row_k = row_k;
-  p.out_off = out_off;
+  p.out_off = out.off;
   p.idx_in = idx_in;
   p.row_ptrs = row_ptrs;
   p.row_len = row_len;
-  p.out_vals = out_vals;
-  p.out_idx = out_idx;
+  p.out_val_ptrs = out.val_ptrs;
+  p.out_idx_ptrs = out.idx_ptrs;
+  p.out_vals = out.vals;
+  p.out_idx = out.idx;
   p.K = K;
   p.R = num_rows;
   p.max_k = max_k;
@@ -159,7 +171,7 @@ int launch(int dtype, const void* vals, const int64_t* row_off, const void* cons
   p.cand_capacity = L.cand_cap;
   p.lds_keys = di.lds_keys;
   p.small_max = di.lds_keys;
-  p.tune = current_tune(di);
+  p.tune = current_tune(di, dtype, flags, num_rows, total_len, max_k);
   const int row_grid = std::min(num_rows, 8192);
   const uint64_t tiles_bound = (uint64_t)total_len / (uint64_t)p.tune.tile + (uint64_t)num_rows;
   const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)p.tune.grid, tiles_bound));
@@ -192,18 +204,25 @@ int rk_topk_batched(int dtype, const void* vals, const int64_t* row_off, const i
                     const int64_t* out_off, void* workspace, size_t ws_bytes, void* stream) {
   if (!valid_dtype(dtype)) return RK_E_UNSUPPORTED;
   if (!row_k || !out_off) return RK_E_ARG;
+  Outputs o;
+  o.vals = out_vals;
+  o.idx = out_idx;
+  o.off = out_off;
   return launch(dtype, vals, row_off, nullptr, nullptr, row_k, 0, num_rows, total_len, max_k,
-                flags, idx_in, out_vals, out_idx, out_off, workspace, ws_bytes, stream);
+                flags, idx_in, o, workspace, ws_bytes, stream);
 }
 
 int rk_topk_rows(int dtype, const void* const* row_ptrs, const int64_t* row_len,
                  const int32_t* row_k, int32_t num_rows, int64_t total_len, int32_t max_k,
-                 uint32_t flags, void* out_vals, int32_t* out_idx, const int64_t* out_off,
+                 uint32_t flags, void* const* out_val_ptrs, int32_t* const* out_idx_ptrs,
                  void* workspace, size_t ws_bytes, void* stream) {
   if (!valid_dtype(dtype)) return RK_E_UNSUPPORTED;
-  if (!row_ptrs || !row_len || !row_k || !out_off) return RK_E_ARG;
+  if (!row_ptrs || !row_len || !row_k || !out_val_ptrs || !out_idx_ptrs) return RK_E_ARG;
+  Outputs o;
+  o.val_ptrs = out_val_ptrs;
+  o.idx_ptrs = out_idx_ptrs;
   return launch(dtype, nullptr, nullptr, row_ptrs, row_len, row_k, 0, num_rows, total_len,
-                max_k, flags, nullptr, out_vals, out_idx, out_off, workspace, ws_bytes, stream);
+                max_k, flags, nullptr, o, workspace, ws_bytes, stream);
 }
 
 int rk_compat_workspace_size(int dtype, int32_t K, int32_t n, int32_t taskNum, uint32_t flags,
@@ -241,9 +260,12 @@ int rk_topk_compat(int dtype, const void* valIn, const int32_t* idxIn, void* val
   if (hipMemcpyAsync(d_off, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice, s) !=
       hipSuccess)
     return RK_E_HIP;
+  Outputs o;
+  o.vals = valOut;
+  o.idx = idxOut;
   return launch(dtype, valIn, d_off, nullptr, nullptr, nullptr, K, taskNum,
-                (int64_t)longest * taskNum, K, flags, idxIn, valOut, idxOut, nullptr,
-                ws + off_bytes, ws_bytes - off_bytes, stream);
+                (int64_t)longest * taskNum, K, flags, idxIn, o, ws + off_bytes,
+                ws_bytes - off_bytes, stream);
 }
 
 int rk_topk_status(const void* workspace, void* stream, uint32_t* status, uint32_t* fallback_rows,

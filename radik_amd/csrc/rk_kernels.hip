@@ -104,14 +104,22 @@ __global__ __launch_bounds__(256) void k_plan_rows(KParams p) {
     if (p.idx_in) idx = p.idx_in + off;
   }
   const int64_t k = p.row_k ? p.row_k[r] : p.K;
-  const int64_t out_pos = p.out_off ? p.out_off[r] : (int64_t)r * p.K;
-  const int64_t slots = p.out_off ? p.out_off[r + 1] - p.out_off[r] : p.K;
-  const int64_t m = len < k ? len : k;
+  int64_t slots;
   RowDesc d;
+  if (p.out_val_ptrs) {  // pointer form: one output block of k slots per row
+    slots = k;
+    d.out_v = p.out_val_ptrs[r];
+    d.out_i = p.out_idx_ptrs[r];
+  } else {
+    const int64_t out_pos = p.out_off ? p.out_off[r] : (int64_t)r * p.K;
+    slots = p.out_off ? p.out_off[r + 1] - p.out_off[r] : p.K;
+    d.out_v = static_cast<U*>(p.out_vals) + out_pos;
+    d.out_i = p.out_idx + out_pos;
+  }
+  const int64_t m = len < k ? len : k;
   d.base = base;
   d.idx_in = idx;
   d.len = len;
-  d.out_pos = out_pos;
   d.k = (int32_t)k;
   d.slots = (int32_t)slots;
   RowState s;
@@ -450,8 +458,8 @@ RK_DEV void sort_and_write(const KParams& p, const RowDesc& d, uint64_t* sbuf, u
   __syncthreads();
   if (P2 > 1) block_bitonic_desc<SEL_NT>(sbuf, P2);
   const U* rb = static_cast<const U*>(d.base);
-  U* ov = static_cast<U*>(p.out_vals) + d.out_pos;
-  int32_t* oi = p.out_idx + d.out_pos;
+  U* ov = static_cast<U*>(d.out_v);
+  int32_t* oi = d.out_i;
   for (uint32_t i = threadIdx.x; i < m; i += SEL_NT) {
     uint32_t sk, pos;
     order_decode<DT>(mode, sbuf[i], sk, pos);
@@ -843,8 +851,8 @@ __global__ __launch_bounds__(STREAM_NT) void k_emit_write(KParams p) {
     const uint32_t rem = p.st[r].rem;
     const uint32_t kk = (uint32_t)d.k;
     const UT* rb = static_cast<const UT*>(d.base);
-    UT* ov = static_cast<UT*>(p.out_vals) + d.out_pos;
-    int32_t* oi = p.out_idx + d.out_pos;
+    UT* ov = static_cast<UT*>(d.out_v);
+    int32_t* oi = d.out_i;
     for (uint32_t t = ta; t < tb; ++t) {
       const uint2 pre = p.tile_pre[p.tile_start[r] + t];
       uint32_t run_g = pre.x, run_e = pre.y;
@@ -911,8 +919,8 @@ __global__ __launch_bounds__(SEL_NT) void k_finalize_ordered(KParams p) {
     if (p.st[r].phase != PH_ORDERED) continue;
     const RowDesc d = p.desc[r];
     const uint32_t m = (uint32_t)d.k;
-    const U* ov = static_cast<const U*>(p.out_vals) + d.out_pos;
-    const int32_t* oi = p.out_idx + d.out_pos;
+    const U* ov = static_cast<const U*>(d.out_v);
+    const int32_t* oi = d.out_i;
     for (uint32_t i = threadIdx.x; i < m; i += SEL_NT)
       sbuf[i] = order_key<DT>(mode, sel_key<DT, LARGEST>((uint32_t)ov[i]), (uint32_t)oi[i]);
     __syncthreads();
@@ -959,6 +967,8 @@ static bool timing_begin(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>& ev) {
   TimingState& t = timing();
   std::lock_guard<std::mutex> lk(t.mu);
   if (!t.on) return false;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
   if (!t.pool.empty()) {
     ev = t.pool.back();
     t.pool.pop_back();

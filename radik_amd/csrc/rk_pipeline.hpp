@@ -35,8 +35,9 @@ enum Status : uint32_t {
 struct RowDesc {
   const void* base;       // first element of the row
   const int32_t* idx_in;  // optional caller indices, parallel to base
+  void* out_v;            // first output value slot of the row
+  int32_t* out_i;         // first output index slot of the row
   int64_t len;
-  int64_t out_pos;  // first output slot
   int32_t k;
   int32_t slots;  // output slots reserved for the row (>= min(len, k))
 };
@@ -80,6 +81,8 @@ struct KParams {
   const int32_t* idx_in;
   const void* const* row_ptrs;  // pointer form (serving)
   const int64_t* row_len;       // pointer form (serving)
+  void* const* out_val_ptrs;    // pointer form: per-row output values
+  int32_t* const* out_idx_ptrs; // pointer form: per-row output indices
   void* out_vals;
   int32_t* out_idx;
   int32_t K;      // k for every row when row_k is null; slot stride when out_off is null
