@@ -128,7 +128,7 @@ int launch(int dtype, const void* vals, const int64_t* row_off, const void* cons
   if (num_rows == 0) return RK_OK;
   if (out.val_ptrs ? !out.idx_ptrs : (!out.vals || !out.idx)) return RK_E_ARG;
   void* out_vals = out.vals;
-  if (row_ptrs ? !row_len : (!vals || !row_off)) return RK_E_ARG;
+  if (row_ptrs ? !row_len : (!row_off || (!vals && total_len > 0))) return RK_E_ARG;
   const DevInfo di = device_info();
   if (!di.ok) return RK_E_HIP;
   if (max_k > di.lds_keys) return RK_E_UNSUPPORTED;

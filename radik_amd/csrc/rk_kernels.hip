@@ -132,8 +132,8 @@ __global__ __launch_bounds__(256) void k_plan_rows(KParams p) {
   if (bad) {
     s.phase = PH_DONE;
     s.flags = 0x80000000u;  // reported by plan_scan
-  } else if (len <= k) {
-    s.phase = PH_COPYALL;
+  } else if (len <= k || k == 0) {
+    s.phase = PH_COPYALL;  // m = min(len, k) elements (none when k == 0)
   } else if (len <= p.small_max) {
     s.phase = PH_SMALL;
   } else {
@@ -292,9 +292,13 @@ __global__ __launch_bounds__(STREAM_NT) void k_filter(KParams p) {
   using UT = typename Traits<DT>::U;
   constexpr int P = Traits<DT>::PACK;
   __shared__ uint64_t stage[STAGE];
-  __shared__ uint32_t stage_cnt;
+  __shared__ uint32_t stage_cnt;   // slots reserved in the stage
+  __shared__ uint32_t stage_fail;  // start of the first reservation that did not fit
   __shared__ uint32_t gbase_sh;
-  if (threadIdx.x == 0) stage_cnt = 0;
+  if (threadIdx.x == 0) {
+    stage_cnt = 0;
+    stage_fail = STAGE;
+  }
   __syncthreads();
   const int lane = lane_id();
   const int wave_base = threadIdx.x & ~63;
@@ -333,7 +337,12 @@ __global__ __launch_bounds__(STREAM_NT) void k_filter(KParams p) {
       const bool to_stage = b + tot <= (uint32_t)STAGE;
       uint32_t gb = 0;
       if (!to_stage) {
-        if (lane == 0) gb = atomicAdd(&p.cand_cnt[r], tot);
+        // Reservations are handed out in increasing order, so the stage holds
+        // exactly the slots below the first one that did not fit.
+        if (lane == 0) {
+          atomicMin(&stage_fail, b);
+          gb = atomicAdd(&p.cand_cnt[r], tot);
+        }
         gb = __shfl(gb, 0, 64);
       }
       uint32_t o = incl - cnt;
@@ -398,14 +407,17 @@ __global__ __launch_bounds__(STREAM_NT) void k_filter(KParams p) {
     }
     // Flush the LDS stage with one global atomic.
     __syncthreads();
-    const uint32_t n = stage_cnt < (uint32_t)STAGE ? stage_cnt : (uint32_t)STAGE;
+    const uint32_t n = stage_cnt < stage_fail ? stage_cnt : stage_fail;
     if (threadIdx.x == 0) gbase_sh = n ? atomicAdd(&p.cand_cnt[r], n) : 0u;
     __syncthreads();
     const uint32_t gb = gbase_sh;
     for (uint32_t i = threadIdx.x; i < n; i += STREAM_NT)
       if (gb + i < cap) p.cand[cbase + gb + i] = stage[i];
     __syncthreads();
-    if (threadIdx.x == 0) stage_cnt = 0;
+    if (threadIdx.x == 0) {
+      stage_cnt = 0;
+      stage_fail = STAGE;
+    }
     __syncthreads();
   });
 }

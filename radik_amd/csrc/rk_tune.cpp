@@ -42,9 +42,6 @@ Key bucket(int dtype, uint32_t flags, int32_t rows, int64_t total, int32_t max_k
   return Key{dtype, (flags & RK_EXACT) ? 1 : 0, lg2ceil(total), lg2ceil(rows), lg2ceil(max_k)};
 }
 
-rk_tune to_c(const rk::Tune& t) {
-  return rk_tune{t.tile, t.grid, t.sample_groups, t.group_elems, t.hist_bits, t.unroll};
-}
 rk::Tune from_c(const rk_tune& t) {
   return rk::Tune{t.tile, t.grid, t.sample_groups, t.group_elems, t.hist_bits, t.unroll};
 }
