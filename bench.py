@@ -163,6 +163,23 @@ def main() -> int:
                      "fallback_rows": st[1], "ordered_rows": st[2]},
     }
 
+    # Achievable read bandwidth of this box: the streaming-read probe over
+    # the same resident values.
+    rk.probe_read(vals)
+    torch.cuda.synchronize()
+    q0 = torch.cuda.Event(enable_timing=True)
+    q1 = torch.cuda.Event(enable_timing=True)
+    q0.record(stream)
+    for _ in range(5):
+        rk.probe_read(vals)
+    q1.record(stream)
+    torch.cuda.synchronize()
+    probe_ms = q0.elapsed_time(q1) / 5
+    probe_gbs = vals.numel() * es / (probe_ms / 1e3) / 1e9
+    result["roofline"]["achievable_peak"] = probe_gbs
+    result["roofline"]["frac_of_achievable"] = achieved / probe_gbs
+    log(f"stream-read probe: {probe_gbs:.0f} GB/s; filter {achieved:.0f} GB/s")
+
     if not args.no_plain and args.plain_steps > 0:
         # One exact multi-pass call per row (RadiK's per-query use of
         # topKRadixSelectL with taskNum = 1, radik/test/test_radik.cu:164-183).

@@ -31,6 +31,12 @@ int rk_gen_fill(int dtype, int kind, uint64_t seed, uint64_t stream_id, int64_t 
                 int64_t row_len, double a, double b, double dup_frac, void* out,
                 void* stream);
 
+/* Streaming-read probe: reads `bytes` (rounded down to 16) from the
+ * 16-byte-aligned device buffer `data` once, with a resident persistent grid,
+ * and writes nothing but (rarely) one word to `out4`.  Timed around, it gives
+ * the achievable HBM read bandwidth of the box. */
+int rk_probe_read(const void* data, int64_t bytes, void* out4, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

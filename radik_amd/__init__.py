@@ -50,7 +50,7 @@ EXPORTED = (
     "rk_get_tune", "rk_set_tune", "rk_gen_fill", "rk_timing_enable", "rk_timing_read",
     "rk_autotune_batched", "rk_tune_cache_save", "rk_tune_cache_load", "rk_tune_cache_clear",
     "rk_tune_cache_size", "rk_graph_begin", "rk_graph_end", "rk_graph_launch",
-    "rk_graph_destroy",
+    "rk_graph_destroy", "rk_probe_read",
 )
 
 
@@ -108,6 +108,8 @@ def lib():
                  "rk_tune_cache_clear", "rk_tune_cache_size", "rk_graph_begin", "rk_graph_end",
                  "rk_graph_launch", "rk_graph_destroy"):
         getattr(L, name).restype = c.c_int
+    L.rk_probe_read.argtypes = [vp, i64, vp, vp]
+    L.rk_probe_read.restype = c.c_int
     L.rk_timing_enable.argtypes = [c.c_int]
     L.rk_timing_read.argtypes = [c.POINTER(c.c_double), c.POINTER(i32)]
     for name in ("rk_timing_enable", "rk_timing_read",
@@ -365,6 +367,14 @@ class _GraphCapture:
             _check(rc, "rk_graph_end")
             self.graph._h = h.value
         return False
+
+
+def probe_read(t, stream=None) -> None:
+    """Launches the streaming-read probe over tensor ``t`` (harness)."""
+    torch = _torch()
+    out = torch.zeros(1, dtype=torch.int32, device=t.device)
+    _check(lib().rk_probe_read(_ptr(t), t.numel() * t.element_size(), _ptr(out),
+                               _stream_ptr(stream)), "rk_probe_read")
 
 
 def get_tune() -> dict:

@@ -103,6 +103,65 @@ RK_DEV float key_threshold_float(uint32_t T) {
   return bits_to_float<DT>(b);
 }
 
+// 16-byte loads through an explicit global (address space 1) pointer, so the
+// compiler emits global_load_dwordx4 rather than flat loads (row bases reach
+// the kernels as generic pointers).  NT selects the non-temporal (streaming)
+// form for data read exactly once.
+typedef unsigned int rk_v4u __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) rk_v4u* rk_gv4;
+template <bool NT>
+RK_DEV uint4 ldg16(const void* p) {
+  rk_v4u v;
+  if constexpr (NT) v = __builtin_nontemporal_load((rk_gv4)p);
+  else v = *((rk_gv4)p);
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+// Buffer loads: one resource descriptor (base + byte range, in SGPRs) per
+// streamed range and 32-bit per-lane byte offsets; the hardware returns 0 for
+// offsets outside the range, so the loads need no guards.  Descriptor word 3
+// of the gfx9 family (incl. gfx950): 0x00020000.
+RK_DEV __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+  // The callers pass workgroup-uniform values; readfirstlane makes that
+  // explicit so the descriptor lives in SGPRs (no per-load waterfall loop).
+  const uint64_t a = (uint64_t)base;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+  const uint32_t n = __builtin_amdgcn_readfirstlane(bytes);
+  void* b = (void*)(((uint64_t)hi << 32) | lo);
+  return __builtin_amdgcn_make_buffer_rsrc(b, (short)0, (int)n, 0x00020000);
+}
+RK_DEV uint4 bload16(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+  const rk_v4u v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)byte_off, 0, 0);
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+// Best value of a 16-byte vector in the selection direction (max for
+// largest-k, min for smallest-k), as fp32.  max/min ignore a NaN operand, so
+// a vector is "all NaN or below the threshold" exactly when this value fails
+// the threshold compare: one compare per vector instead of one per element.
+template <int DT, bool LARGEST>
+RK_DEV float vec_best(const uint4& x) {
+  auto pick = [](float a, float b) { return LARGEST ? fmaxf(a, b) : fminf(a, b); };
+  if constexpr (DT == F32) {
+    return pick(pick(__uint_as_float(x.x), __uint_as_float(x.y)),
+                pick(__uint_as_float(x.z), __uint_as_float(x.w)));
+  } else if constexpr (DT == BF16) {
+    const float a = pick(__uint_as_float(x.x << 16), __uint_as_float(x.x & 0xffff0000u));
+    const float b = pick(__uint_as_float(x.y << 16), __uint_as_float(x.y & 0xffff0000u));
+    const float c = pick(__uint_as_float(x.z << 16), __uint_as_float(x.z & 0xffff0000u));
+    const float d = pick(__uint_as_float(x.w << 16), __uint_as_float(x.w & 0xffff0000u));
+    return pick(pick(a, b), pick(c, d));
+  } else {
+    auto h = [](uint32_t u) { return (float)__builtin_bit_cast(_Float16, (uint16_t)u); };
+    const float a = pick(h(x.x & 0xffffu), h(x.x >> 16));
+    const float b = pick(h(x.y & 0xffffu), h(x.y >> 16));
+    const float c = pick(h(x.z & 0xffffu), h(x.z >> 16));
+    const float d = pick(h(x.w & 0xffffu), h(x.w >> 16));
+    return pick(pick(a, b), pick(c, d));
+  }
+}
+
 // Unpacked 16-byte vector of PACK elements (bit patterns, low element first).
 template <int DT>
 struct Vec {

@@ -93,7 +93,44 @@ __global__ void k_gen(int dtype, int kind, uint64_t base, int64_t n, int64_t row
   }
 }
 
+// Streaming-read probe: every workgroup reads one contiguous slice with
+// 16-byte loads (8 in flight per lane) and folds it into one word, so the
+// kernel moves exactly `bytes` from HBM: the achievable read bandwidth the
+// top-k filter is compared against.
+__global__ __launch_bounds__(256) void k_probe_read(const uint4* p, int64_t nvec,
+                                                    unsigned int* out) {
+  const int64_t G = gridDim.x;
+  const int64_t v0 = nvec * blockIdx.x / G, v1 = nvec * (blockIdx.x + 1) / G;
+  uint32_t acc = 0;
+  for (int64_t j = v0 + threadIdx.x; j < v1; j += 256 * 8) {
+    uint4 x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t jj = j + u * 256;
+      x[u] = jj < v1 ? p[jj] : make_uint4(0u, 0u, 0u, 0u);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc ^= x[u].x ^ x[u].y ^ x[u].z ^ x[u].w;
+  }
+  if (acc == 0x9E3779B9u) atomicXor(out, acc);  // keeps the loads live
+}
+
 }  // namespace
+
+extern "C" int rk_probe_read(const void* data, int64_t bytes, void* out4, void* stream) {
+  if (!data || !out4 || bytes < 0 || ((uintptr_t)data % 16)) return RK_E_ARG;
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return RK_E_HIP;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return RK_E_HIP;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_probe_read, 256, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  hipLaunchKernelGGL(k_probe_read, dim3(cus * per_cu), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), static_cast<const uint4*>(data), bytes / 16,
+                     static_cast<unsigned int*>(out4));
+  return hipGetLastError() == hipSuccess ? RK_OK : RK_E_HIP;
+}
 
 extern "C" int rk_gen_fill(int dtype, int kind, uint64_t seed, uint64_t stream_id, int64_t n,
                            int64_t row_len, double a, double b, double dup_frac, void* out,

@@ -38,7 +38,11 @@ constexpr int SEL_NT = 1024;         // threads of the per-row select kernels
 constexpr int LDS_KEYS_MAX = 16384;  // compile-time bound of the LDS key buffer
 constexpr int SEL_ITEMS = LDS_KEYS_MAX / SEL_NT;
 constexpr int STREAM_NT = 256;       // threads of the streaming kernels
-constexpr int STAGE = 2048;          // LDS candidate staging entries per workgroup
+// The streaming filter reads each key exactly once: non-temporal loads keep
+// it from evicting the row metadata and candidate buffers from the caches.
+#ifndef RK_FILTER_NT
+#define RK_FILTER_NT 1
+#endif
 
 // ---------------------------------------------------------------------------
 // Work partition of the streaming kernels: the tiles of all rows form one
@@ -126,6 +130,7 @@ __global__ __launch_bounds__(256) void k_plan_rows(KParams p) {
   s.flags = 0;
   s.T = s.prefix = s.resolved = s.rem = s.V = 0;
   s.cap = 0;
+  s.why = FB_NONE;
   uint32_t tiles = 0;
   const bool bad = len < 0 || k < 0 || k > p.max_k || m > slots || slots > 0x7fffffff ||
                    len > 0x7fffffffLL;
@@ -244,7 +249,9 @@ __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
     int64_t ng = p.tune.sample_groups;
     const int64_t max_keys = (int64_t)p.lds_keys * 2;
     if (ng * ge > max_keys) ng = max_keys / ge;
-    if (ng * ge * 64 > len) ng = len / (64 * ge);  // read at most 1/64 of the row
+    if (ng * ge * 64 > len) ng = len / (64 * ge);  // read about 1/64 of the row,
+    const int64_t ng_min = len / (2 * ge) < 64 ? len / (2 * ge) : 64;
+    if (ng < ng_min) ng = ng_min;                   // but at least 64 runs
     if (ng < 1) ng = 1;
     const uint32_t s = (uint32_t)(ng * ge);
     for (uint32_t i = threadIdx.x; i < s; i += SEL_NT) {
@@ -266,9 +273,10 @@ __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
       RowState& st = p.st[r];
       // Expected candidates: the sampled fraction at or above T, scaled.
       const double est = (double)ge_cnt * (double)len / (double)s;
-      if (T == 0 || rank >= s || est * 1.25 + 1024.0 > (double)st.cap) {
+      if (T == 0 || rank >= s || est * 1.25 > (double)st.cap) {
         st.phase = PH_HIST;
         st.flags = RF_NEED_INIT;
+        st.why = (T == 0 || rank >= s) ? FB_SAMPLE_DEGENERATE : FB_SAMPLE_TOO_MANY;
         atomicAdd(&p.hdr->fallback_rows, 1u);
       } else {
         st.T = T;
@@ -287,21 +295,72 @@ __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
 // per segment.  The compare is done on the float value (one VALU op per
 // element); it admits every key >= T plus possibly -0/+0 neighbours.
 // ---------------------------------------------------------------------------
+// Candidate staging of the streaming filter.  Each wave owns a private LDS
+// stage of WSTAGE composite keys; only that wave writes or reads it (LDS
+// operations of one wave complete in order), so appends need no atomics and
+// a full stage is flushed by the wave alone with one global atomic on the
+// row's counter: no workgroup barrier anywhere in the streaming loop.
+constexpr int WSTAGE = 512;
+
+struct AppendCtx {
+  uint64_t* stage;     // this wave's LDS stage (WSTAGE entries)
+  uint64_t* cand;      // global candidate slots of the row
+  uint32_t* cand_cnt;  // global candidate counter of the row
+  uint32_t cap;        // candidate slots of the row
+};
+
+// Moves the wave's staged keys to the row's candidate slots.
+RK_DEV void wave_flush(const AppendCtx& ac, uint32_t& n) {
+  if (n == 0) return;
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  uint32_t gb = 0;
+  if (lane_id() == 0) gb = atomicAdd(ac.cand_cnt, n);
+  gb = __shfl(gb, 0, 64);
+  for (uint32_t i = lane_id(); i < n; i += 64)
+    if (gb + i < ac.cap) ac.cand[gb + i] = ac.stage[i];
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  n = 0;
+}
+
+// Wave-wide append of the masked elements of one 16-byte vector per lane
+// (called by the whole wave, under a wave-uniform branch).  Slots come from
+// one ballot per element position: position c of lane l goes to
+//   sum over c' < c of popcount(ballot_c') + popcount(ballot_c below lane l).
+template <int DT, bool LARGEST>
+RK_DEV void append_vec(const AppendCtx& ac, uint32_t& n, uint4 x, uint32_t mask, int64_t first) {
+  constexpr int P = Traits<DT>::PACK;
+  const Vec<DT> v = unpack<DT>(x);
+  uint64_t bal[P];
+  uint32_t tot = 0;
+#pragma unroll
+  for (int c = 0; c < P; ++c) {
+    bal[c] = __ballot((mask >> c) & 1u);
+    tot += (uint32_t)__popcll(bal[c]);
+  }
+  if (tot == 0) return;
+  if (n + tot > (uint32_t)WSTAGE) wave_flush(ac, n);  // tot <= 64 * P <= WSTAGE
+  uint32_t base = n;
+#pragma unroll
+  for (int c = 0; c < P; ++c) {
+    if ((mask >> c) & 1u) {
+      const uint32_t pos = (uint32_t)(first + c);
+      ac.stage[base + mbcnt(bal[c])] =
+          ((uint64_t)sel_key<DT, LARGEST>(v.e[c]) << 32) | (uint32_t)(~pos);
+    }
+    base += (uint32_t)__popcll(bal[c]);
+  }
+  n += tot;
+}
+
 template <int DT, bool LARGEST, int U>
 __global__ __launch_bounds__(STREAM_NT) void k_filter(KParams p) {
   using UT = typename Traits<DT>::U;
   constexpr int P = Traits<DT>::PACK;
-  __shared__ uint64_t stage[STAGE];
-  __shared__ uint32_t stage_cnt;   // slots reserved in the stage
-  __shared__ uint32_t stage_fail;  // start of the first reservation that did not fit
-  __shared__ uint32_t gbase_sh;
-  if (threadIdx.x == 0) {
-    stage_cnt = 0;
-    stage_fail = STAGE;
-  }
-  __syncthreads();
+  __shared__ uint64_t stage[STREAM_NT / 64][WSTAGE];
   const int lane = lane_id();
-  const int wave_base = threadIdx.x & ~63;
+  // wave-uniform (readfirstlane tells the compiler), so the streaming loop's
+  // control flow stays scalar and its loads stay unconditional
+  const int wave_base = __builtin_amdgcn_readfirstlane(threadIdx.x & ~63);
 
   for_each_segment(p, [&](int r, uint32_t ta, uint32_t tb) {
     if (p.st[r].phase != PH_FILTER) return;  // uniform across the block
@@ -325,100 +384,87 @@ __global__ __launch_bounds__(STREAM_NT) void k_filter(KParams p) {
       const float x = bits_to_float<DT>(u);
       return LARGEST ? (x >= tf) : (x <= tf);
     };
-    // Wave-wide append of the masked elements of one vector per lane.
-    auto append = [&](const uint32_t* e, uint32_t mask, int64_t first) {
-      const uint32_t cnt = __popc(mask);
-      const uint32_t incl = wave_incl_scan(cnt);
-      const uint32_t tot = __shfl(incl, 63, 64);
-      if (tot == 0) return;
-      uint32_t b = 0;
-      if (lane == 0) b = atomicAdd(&stage_cnt, tot);
-      b = __shfl(b, 0, 64);
-      const bool to_stage = b + tot <= (uint32_t)STAGE;
-      uint32_t gb = 0;
-      if (!to_stage) {
-        // Reservations are handed out in increasing order, so the stage holds
-        // exactly the slots below the first one that did not fit.
-        if (lane == 0) {
-          atomicMin(&stage_fail, b);
-          gb = atomicAdd(&p.cand_cnt[r], tot);
-        }
-        gb = __shfl(gb, 0, 64);
-      }
-      uint32_t o = incl - cnt;
-#pragma unroll
-      for (int c = 0; c < P; ++c) {
-        if ((mask >> c) & 1u) {
-          const uint32_t pos = (uint32_t)(first + c);
-          const uint64_t key = ((uint64_t)sel_key<DT, LARGEST>(e[c]) << 32) | (uint32_t)(~pos);
-          if (to_stage) stage[b + o] = key;
-          else if (gb + o < cap) p.cand[cbase + gb + o] = key;
-          ++o;
-        }
-      }
-    };
+    AppendCtx ac;
+    ac.stage = stage[wave_base >> 6];
+    ac.cand = p.cand + cbase;
+    ac.cand_cnt = &p.cand_cnt[r];
+    ac.cap = cap;
+    uint32_t staged = 0;  // keys in this wave's stage (wave-uniform)
 
     // Head and tail elements outside the aligned body: wave 0, one per lane.
     if (wave_base == 0) {
-      uint32_t e[P];
-#pragma unroll
-      for (int c = 0; c < P; ++c) e[c] = 0;
       int64_t pos = -1;
       if (lane < P && es + lane < al) pos = es + lane;
       if (lane >= 32 && lane - 32 < P && tail0 + (lane - 32) < ee) pos = tail0 + (lane - 32);
       uint32_t mask = 0;
+      uint4 x = make_uint4(0u, 0u, 0u, 0u);
       if (pos >= 0) {
-        e[0] = (uint32_t)rb[pos];
-        mask = pred(e[0]) ? 1u : 0u;
+        const uint32_t u0 = (uint32_t)rb[pos];
+        x.x = u0;  // element 0 of the vector (16-bit types: low half)
+        mask = pred(u0) ? 1u : 0u;
       }
-      if (__ballot(mask != 0)) append(e, mask, pos);
+      if (__ballot(mask != 0)) append_vec<DT, LARGEST>(ac, staged, x, mask, pos);
     }
     // Aligned body.
     const uint4* vp = reinterpret_cast<const uint4*>(rb + al);
-    for (int64_t jb = wave_base; jb < nv; jb += (int64_t)STREAM_NT * U) {
-      uint4 x[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int64_t j = jb + lane + (int64_t)u * STREAM_NT;
-        x[u] = j < nv ? vp[j] : make_uint4(0u, 0u, 0u, 0u);
+    if (all) {
+      // Threshold admits every key (kept simple: such rows are rare).
+      for (int64_t jb = wave_base; jb < nv; jb += STREAM_NT) {
+        const int64_t j = jb + lane;
+        uint4 x = make_uint4(0u, 0u, 0u, 0u);
+        uint32_t m = 0;
+        if (j < nv) {
+          x = ldg16<false>(vp + j);
+          m = (1u << P) - 1u;
+        }
+        append_vec<DT, LARGEST>(ac, staged, x, m, al + j * P);
       }
-      uint32_t m[U];
-      uint32_t anym = 0;
+    } else {
+      // Buffer loads, U x 16 bytes per lane per step.  Ranges are cut into
+      // chunks of at most 2^30 bytes for the 32-bit offsets.
+      constexpr uint32_t STEP = (uint32_t)STREAM_NT * U;
+      constexpr int64_t CHUNK = int64_t(1) << 26;  // vectors per chunk
+      for (int64_t c0 = 0; c0 < nv; c0 += CHUNK) {
+        const uint32_t nvc = (uint32_t)((nv - c0) < CHUNK ? (nv - c0) : CHUNK);
+        const __amdgpu_buffer_rsrc_t rs = make_rsrc(vp + c0, nvc * 16u);
+        auto load_set = [&](uint4 (&x)[U], uint32_t jb) {
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int64_t j = jb + lane + (int64_t)u * STREAM_NT;
-        const Vec<DT> v = unpack<DT>(x[u]);
-        uint32_t mm = 0;
+          for (int u = 0; u < U; ++u) x[u] = bload16(rs, (jb + lane + u * STREAM_NT) * 16u);
+        };
+        auto test_set = [&](const uint4 (&x)[U], uint32_t jb) {
+          bool hit = false;
 #pragma unroll
-        for (int c = 0; c < P; ++c) mm |= (pred(v.e[c]) ? 1u : 0u) << c;
-        m[u] = j < nv ? mm : 0u;
-        anym |= m[u];
-      }
-      if (__ballot(anym != 0)) {
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          if (__ballot(m[u] != 0)) {
-            const Vec<DT> v = unpack<DT>(x[u]);
-            const int64_t j = jb + lane + (int64_t)u * STREAM_NT;
-            append(v.e, m[u], al + j * P);
+          for (int u = 0; u < U; ++u) {
+            const float b = vec_best<DT, LARGEST>(x[u]);
+            hit |= LARGEST ? (b >= tf) : (b <= tf);
           }
+          if (__ballot(hit)) {  // rare: exact per-element masks
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+              const uint32_t j = jb + lane + u * STREAM_NT;
+              const Vec<DT> v = unpack<DT>(x[u]);
+              uint32_t mm = 0;
+#pragma unroll
+              for (int c = 0; c < P; ++c) {
+                const float xf = bits_to_float<DT>(v.e[c]);
+                mm |= ((LARGEST ? (xf >= tf) : (xf <= tf)) ? 1u : 0u) << c;
+              }
+              mm = j < nvc ? mm : 0u;
+              if (__ballot(mm != 0)) append_vec<DT, LARGEST>(ac, staged, x[u], mm, al + (c0 + j) * P);
+            }
+          }
+        };
+        // Loads past the range return zeros (masked above), so they are
+        // issued unconditionally; U loads per lane are in flight per wave
+        // and 8 waves per SIMD hide the latency.
+        for (uint32_t jb = (uint32_t)wave_base; jb < nvc; jb += STEP) {  // wave-uniform
+          uint4 x[U];
+          load_set(x, jb);
+          test_set(x, jb);
         }
       }
     }
-    // Flush the LDS stage with one global atomic.
-    __syncthreads();
-    const uint32_t n = stage_cnt < stage_fail ? stage_cnt : stage_fail;
-    if (threadIdx.x == 0) gbase_sh = n ? atomicAdd(&p.cand_cnt[r], n) : 0u;
-    __syncthreads();
-    const uint32_t gb = gbase_sh;
-    for (uint32_t i = threadIdx.x; i < n; i += STREAM_NT)
-      if (gb + i < cap) p.cand[cbase + gb + i] = stage[i];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      stage_cnt = 0;
-      stage_fail = STAGE;
-    }
-    __syncthreads();
+    wave_flush(ac, staged);
   });
 }
 
@@ -531,6 +577,7 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
         n = p.cand_cnt[r];
         const uint64_t* src = p.cand + p.cand_base[r];
         bool ok = n <= st.cap && n >= k;
+        uint32_t why = n > st.cap ? FB_OVERFLOW : FB_TOO_FEW;
         if (ok) {
           in_lds = n <= (uint32_t)p.lds_keys;
           if (in_lds) {
@@ -550,6 +597,7 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
             if (st.flags & RF_EXACT_T) atomicOr(&p.hdr->status, (uint32_t)ST_INTERNAL);
             p.st[r].phase = PH_HIST;
             p.st[r].flags = RF_NEED_INIT;
+            p.st[r].why = why;
             atomicAdd(&p.hdr->fallback_rows, 1u);
           }
           __syncthreads();
@@ -682,7 +730,7 @@ __global__ __launch_bounds__(STREAM_NT) void k_hist(KParams p) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int64_t j = jb + lane + (int64_t)u * STREAM_NT;
-        x[u] = j < nv ? vp[j] : make_uint4(0u, 0u, 0u, 0u);
+        x[u] = j < nv ? ldg16<false>(vp + j) : make_uint4(0u, 0u, 0u, 0u);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -772,7 +820,7 @@ RK_DEV uint32_t load_slot(const typename Traits<DT>::U* rb, int64_t es, int64_t 
   constexpr int P = Traits<DT>::PACK;
   if (s >= 1 && s <= nv) {
     first = al + (s - 1) * P;
-    v = unpack<DT>(*reinterpret_cast<const uint4*>(rb + first));
+    v = unpack<DT>(ldg16<false>(rb + first));
     return (1u << P) - 1u;
   }
   int64_t lo, hi;
@@ -998,12 +1046,36 @@ static void timing_end(hipStream_t s, const std::pair<hipEvent_t, hipEvent_t>& e
   t.pending.push_back(ev);
 }
 
+// Workgroups of a persistent kernel that can be resident at once on the
+// device: the static partition needs every workgroup running from the start.
+template <typename K>
+static int resident_grid(K kernel, int threads) {
+  static std::mutex mu;
+  static std::vector<std::pair<const void*, int>> memo;
+  const void* key = reinterpret_cast<const void*>(kernel);
+  std::lock_guard<std::mutex> lk(mu);
+  for (auto& m : memo)
+    if (m.first == key) return m.second;
+  int dev = 0, cus = 0, per_cu = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  const int g = cus * per_cu;
+  memo.emplace_back(key, g);
+  return g;
+}
+
 template <int DT, bool LARGEST, int U>
-static hipError_t run_typed_u(const KParams& p, const Launch& L) {
+static hipError_t run_typed_u(const KParams& p, const Launch& L0) {
   constexpr int KB = Traits<DT>::KEY_BITS;
   const size_t dyn = (size_t)p.lds_keys * sizeof(uint64_t);
   const int rows_blocks = (p.R + 255) / 256;
-  hipStream_t s = L.stream;
+  hipStream_t s = L0.stream;
+  Launch L = L0;
+  L.grid = std::min(L0.grid, resident_grid(k_filter<DT, LARGEST, U>, STREAM_NT));
+  const int hist_grid = std::min(L0.grid, resident_grid(k_hist<DT, LARGEST, U>, STREAM_NT));
   k_plan_rows<DT><<<rows_blocks, 256, 0, s>>>(p);
   k_plan_scan<<<1, 1024, 0, s>>>(p);
   if (!(p.flags & FL_EXACT)) {
@@ -1017,7 +1089,7 @@ static hipError_t run_typed_u(const KParams& p, const Launch& L) {
   k_exact_init<<<L.row_grid, 256, 0, s>>>(p);
   const int passes = (KB + p.tune.hist_bits - 1) / p.tune.hist_bits;
   for (int i = 0; i < passes; ++i) {
-    k_hist<DT, LARGEST, U><<<L.grid, STREAM_NT, 0, s>>>(p);
+    k_hist<DT, LARGEST, U><<<hist_grid, STREAM_NT, 0, s>>>(p);
     k_hist_select<<<L.row_grid, 256, 0, s>>>(p, KB);
   }
   k_filter<DT, LARGEST, U><<<L.grid, STREAM_NT, 0, s>>>(p);

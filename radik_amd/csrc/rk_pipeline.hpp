@@ -51,6 +51,15 @@ struct RowState {
   uint32_t rem;       // PH_HIST / PH_ORDERED: elements still to take at the boundary
   uint32_t V;         // PH_ORDERED: the k-th key
   uint32_t cap;       // candidate capacity of the row
+  uint32_t why;       // why the row left the sampled path (diagnostics, Fallback)
+};
+
+enum Fallback : uint32_t {
+  FB_NONE = 0,
+  FB_SAMPLE_DEGENERATE = 1,  // threshold would admit every key (NaN-heavy / rank >= sample)
+  FB_SAMPLE_TOO_MANY = 2,    // estimated candidates exceed the row's slots (heavy ties)
+  FB_OVERFLOW = 3,           // more candidates than slots arrived
+  FB_TOO_FEW = 4,            // fewer than k candidates were >= T (unrepresentative sample)
 };
 
 struct Header {

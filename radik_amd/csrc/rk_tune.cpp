@@ -86,7 +86,7 @@ int rk_autotune_batched(int dtype, const void* vals, const int64_t* row_off,
   std::vector<rk_tune> cands;
   for (int tile : {4096, 16384, 65536})
     for (int gm : {4, 8, 16})
-      for (int un : {4, 8})
+      for (int un : {2, 4})
         for (int sg : {512, 2048})
           cands.push_back(rk_tune{tile, gm * cus, sg, 16, saved.hist_bits, un});
   float best_t = 1e30f;
