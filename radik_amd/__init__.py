@@ -34,7 +34,7 @@ RK_LARGEST, RK_SORTED, RK_ASCEND, RK_EXACT = 1, 2, 4, 8
 GEN_UNIFORM, GEN_NORMAL, GEN_ZIPF, GEN_NEARCONST, GEN_RANGE, GEN_CONST = range(6)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "lib", "libradik_amd.so")
+_LIB_PATH = os.environ.get("RADIK_AMD_LIB") or os.path.join(_HERE, "lib", "libradik_amd.so")
 _lib = None
 
 _ERRORS = {

@@ -59,7 +59,7 @@ DevInfo device_info() {
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct Layout {
-  size_t hdr, desc, st, tile_start, cand_base, cand_cnt, ghist, tile_cnt, tile_pre, cand, total;
+  size_t hdr, desc, st, tile_start, cand_base, cand_cnt, order, ghist, tile_cnt, tile_pre, cand, total;
   uint64_t tile_cap, cand_cap;
 };
 
@@ -80,6 +80,7 @@ Layout make_layout(int32_t R, int64_t total_len, int32_t max_k) {
   L.tile_start = o; o = align256(o + (Ru + 1) * sizeof(uint32_t));
   L.cand_base = o; o = align256(o + (Ru + 1) * sizeof(uint64_t));
   L.cand_cnt = o; o = align256(o + Ru * sizeof(uint32_t));
+  L.order = o; o = align256(o + Ru * sizeof(uint32_t));
   L.ghist = o; o = align256(o + Ru * rk::HIST_MAX_BINS * sizeof(uint32_t));
   L.tile_cnt = o; o = align256(o + L.tile_cap * sizeof(uint2));
   L.tile_pre = o; o = align256(o + L.tile_cap * sizeof(uint2));
@@ -163,15 +164,23 @@ int launch(int dtype, const void* vals, const int64_t* row_off, const void* cons
   p.tile_start = reinterpret_cast<uint32_t*>(ws + L.tile_start);
   p.cand_base = reinterpret_cast<uint64_t*>(ws + L.cand_base);
   p.cand_cnt = reinterpret_cast<uint32_t*>(ws + L.cand_cnt);
+  p.order = reinterpret_cast<uint32_t*>(ws + L.order);
   p.ghist = reinterpret_cast<uint32_t*>(ws + L.ghist);
   p.tile_cnt = reinterpret_cast<uint2*>(ws + L.tile_cnt);
   p.tile_pre = reinterpret_cast<uint2*>(ws + L.tile_pre);
   p.cand = reinterpret_cast<uint64_t*>(ws + L.cand);
   p.tile_capacity = L.tile_cap;
   p.cand_capacity = L.cand_cap;
-  p.lds_keys = di.lds_keys;
-  p.small_max = di.lds_keys;
+  // LDS of the select kernels sized to this call: the k selected keys (and
+  // rows up to that length) live in LDS; larger candidate sets are selected
+  // from global memory.  Smaller buffers let more workgroups share a CU.
+  int lk = 4096;
+  while (lk < max_k) lk <<= 1;
+  p.lds_keys = std::min(lk, di.lds_keys);
+  p.small_max = p.lds_keys;
   p.tune = current_tune(di, dtype, flags, num_rows, total_len, max_k);
+  p.sample_keys = std::min(16384, p.tune.sample_groups * p.tune.group_elems);
+  p.round = 0;
   const int row_grid = std::min(num_rows, 8192);
   const uint64_t tiles_bound = (uint64_t)total_len / (uint64_t)p.tune.tile + (uint64_t)num_rows;
   const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)p.tune.grid, tiles_bound));

@@ -103,6 +103,19 @@ RK_DEV float key_threshold_float(uint32_t T) {
   return bits_to_float<DT>(b);
 }
 
+// Scalar loads through explicit address spaces.  Generic pointers may alias
+// LDS, which stops the compiler from moving a global load above an earlier
+// LDS store or atomic; naming the address space lets loops keep several
+// global loads in flight.
+template <typename T>
+RK_DEV T ldg(const T* p) {
+  return *((const __attribute__((address_space(1))) T*)p);
+}
+template <typename T>
+RK_DEV T ld_lds(const T* p) {
+  return *((const __attribute__((address_space(3))) T*)p);
+}
+
 // 16-byte loads through an explicit global (address space 1) pointer, so the
 // compiler emits global_load_dwordx4 rather than flat loads (row bases reach
 // the kernels as generic pointers).  NT selects the non-temporal (streaming)
@@ -302,17 +315,31 @@ RK_DEV void block_find_crossing(const uint32_t* hist, int nbins, uint32_t k,
 //   #{i : keys[i] >= K} >= k  and  #{i : keys[i] > K'} < k for K' >= K in
 // the selected bin, i.e. keys >= K are exactly the k largest when the keys
 // are distinct (composite keys), and at least the k largest otherwise.
-// `keys` may point to LDS or global memory.  hist: LDS, 1 << HB words.
+// `keys` points to global memory (GLOBAL) or LDS.  hist: LDS, 1 << HB words.
 // The scan starts below the common prefix of all keys (the key-space form
 // of RadiK's adaptive scaling): bits every key shares are never histogrammed.
-template <int NT, int HB, typename KeyT>
-RK_DEV KeyT block_radix_select(const KeyT* keys, uint32_t n, uint32_t k,
-                               uint32_t* hist, uint32_t* scratch,
-                               uint64_t* scratch64, uint32_t* out) {
+// Loads are issued UN at a time before their LDS atomics.
+template <int NT, int HB, typename KeyT, bool GLOBAL>
+RK_DEV KeyT block_radix_select(const KeyT* keys, uint32_t n, uint32_t k, uint32_t* hist,
+                               uint32_t* scratch, uint64_t* scratch64, uint32_t* out) {
   constexpr int BITS = (int)sizeof(KeyT) * 8;
-  const KeyT k0 = keys[0];
+  constexpr int UN = 4;
+  auto ld = [&](uint32_t i) -> KeyT {
+    if constexpr (GLOBAL) return ldg(keys + i);
+    else return ld_lds(keys + i);
+  };
+  const KeyT k0 = ld(0);
   uint64_t diff = 0;
-  for (uint32_t i = threadIdx.x; i < n; i += NT) diff |= (uint64_t)(keys[i] ^ k0);
+  for (uint32_t i0 = threadIdx.x; i0 < n; i0 += NT * UN) {
+    KeyT x[UN];
+#pragma unroll
+    for (int u = 0; u < UN; ++u) {
+      const uint32_t i = i0 + u * NT;
+      x[u] = i < n ? ld(i) : k0;
+    }
+#pragma unroll
+    for (int u = 0; u < UN; ++u) diff |= (uint64_t)(x[u] ^ k0);
+  }
   diff = block_or64<NT>(diff, scratch64);
   if (diff == 0) return k0;  // all keys equal: every key is >= k0
   int lo = 64 - __builtin_clzll(diff);  // number of low bits still unresolved
@@ -324,10 +351,19 @@ RK_DEV KeyT block_radix_select(const KeyT* keys, uint32_t n, uint32_t k,
     const uint32_t nb = 1u << w;
     for (uint32_t i = threadIdx.x; i < nb; i += NT) hist[i] = 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += NT) {
-      const KeyT x = keys[i];
-      const bool match = (lo >= BITS) || ((x >> lo) == (prefix >> lo));
-      if (match) atomicAdd(&hist[(uint32_t)(x >> shift) & (nb - 1)], 1u);
+    for (uint32_t i0 = threadIdx.x; i0 < n; i0 += NT * UN) {
+      KeyT x[UN];
+#pragma unroll
+      for (int u = 0; u < UN; ++u) {
+        const uint32_t i = i0 + u * NT;
+        x[u] = i < n ? ld(i) : (KeyT)0;
+      }
+#pragma unroll
+      for (int u = 0; u < UN; ++u) {
+        const uint32_t i = i0 + u * NT;
+        const bool match = (lo >= BITS) || ((x[u] >> lo) == (prefix >> lo));
+        if (i < n && match) atomicAdd(&hist[(uint32_t)(x[u] >> shift) & (nb - 1)], 1u);
+      }
     }
     __syncthreads();
     block_find_crossing<NT>(hist, (int)nb, rem, scratch, out);
@@ -342,9 +378,79 @@ RK_DEV KeyT block_radix_select(const KeyT* keys, uint32_t n, uint32_t k,
   return prefix;
 }
 
-// In-place bitonic sort of P2 (power of two) 64-bit keys in LDS, descending.
+// Descending bitonic sort of P2 (power of two) 64-bit keys held in LDS.
+// The keys are spread over as few waves as possible: thread t owns E
+// consecutive keys in registers (nthr = P2 / E threads, whole waves except
+// when P2 < 64 E).  A compare-exchange at distance `stride` runs
+//   in registers      when stride < E,
+//   by lane shuffles  when the partner is in the same wave (stride / E < 64),
+//   through LDS       otherwise (the only stages with a workgroup barrier).
+// Waves holding no keys skip the register and shuffle stages, so a sort of
+// up to 64 E keys is done by one wave with no barrier at all.
+template <int NT, int E>
+RK_DEV void bitonic_sort_e(uint64_t* a, uint32_t P2) {
+  const uint32_t t = threadIdx.x;
+  const uint32_t nthr = P2 / E;  // threads holding keys
+  const bool wave_busy = (t & ~63u) < nthr;
+  uint64_t v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = t < nthr ? a[t * E + e] : 0ull;
+  for (uint32_t size = 2; size <= P2; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride < (uint32_t)E) {
+        // compile-time register indices only (a runtime index would move v[]
+        // to scratch memory)
+#pragma unroll
+        for (int sc = E / 2; sc >= 1; sc >>= 1) {
+          if (stride == (uint32_t)sc) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+              if (!(e & sc)) {
+                const uint32_t i = t * E + e;
+                const uint64_t x = v[e], y = v[e + sc];
+                const bool desc = !(i & size);
+                const bool sw = desc ? (x < y) : (x > y);
+                v[e] = sw ? y : x;
+                v[e + sc] = sw ? x : y;
+              }
+            }
+          }
+        }
+      } else {
+        const uint32_t ls = stride / E;  // distance in threads
+        if (ls >= 64) {
+#pragma unroll
+          for (int e = 0; e < E; ++e)
+            if (t < nthr) a[t * E + e] = v[e];
+          __syncthreads();
+        }
+        if (ls >= 64 || wave_busy) {
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+            uint64_t y;
+            if (ls < 64) y = __shfl_xor(v[e], (int)ls, 64);
+            else y = (t < nthr) ? a[(t ^ ls) * E + e] : 0ull;
+            const uint32_t i = t * E + e;
+            const bool lower = !(i & stride);
+            const bool desc = !(i & size);
+            const bool keep_max = (lower == desc);
+            v[e] = keep_max ? (v[e] > y ? v[e] : y) : (v[e] < y ? v[e] : y);
+          }
+        }
+        if (ls >= 64) __syncthreads();
+      }
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    if (t < nthr) a[t * E + e] = v[e];
+  __syncthreads();
+}
+
+// Plain LDS bitonic sort (every stage through LDS), for the largest sizes
+// where keeping E keys per thread in registers would spill.
 template <int NT>
-RK_DEV void block_bitonic_desc(uint64_t* a, uint32_t P2) {
+RK_DEV void bitonic_sort_lds(uint64_t* a, uint32_t P2) {
   for (uint32_t size = 2; size <= P2; size <<= 1) {
     for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
       for (uint32_t i = threadIdx.x; i < (P2 >> 1); i += NT) {
@@ -360,6 +466,16 @@ RK_DEV void block_bitonic_desc(uint64_t* a, uint32_t P2) {
       __syncthreads();
     }
   }
+}
+
+// Measured on gfx950 (one workgroup per CU): a single wave sorts up to 64
+// keys faster with shuffles than the whole workgroup does through LDS; from
+// 128 keys on, the dependent shuffle chain loses to LDS stages, whose
+// barriers the other waves' work overlaps.
+template <int NT>
+RK_DEV void block_bitonic_desc(uint64_t* a, uint32_t P2) {
+  if (P2 <= 64) bitonic_sort_e<NT, 1>(a, P2);
+  else bitonic_sort_lds<NT>(a, P2);
 }
 
 RK_DEV uint32_t next_pow2(uint32_t x) {

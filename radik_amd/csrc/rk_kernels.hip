@@ -34,9 +34,20 @@
 
 namespace rk {
 
-constexpr int SEL_NT = 1024;         // threads of the per-row select kernels
+// Optional phase timing for tuning builds (-DRK_PHASE_TIMING): wall-clock
+// stamps per row into that row's (otherwise idle) histogram slot.
+#ifdef RK_PHASE_TIMING
+#define RK_STAMP(p, r, slot)                                                     \
+  do {                                                                           \
+    if (threadIdx.x == 0) (p).ghist[(size_t)(r) * HIST_MAX_BINS + 64 + (slot)] = \
+        (uint32_t)wall_clock64();                                                \
+  } while (0)
+#else
+#define RK_STAMP(p, r, slot) do {} while (0)
+#endif
+
+constexpr int SEL_NT = 512;          // threads of the per-row select kernels
 constexpr int LDS_KEYS_MAX = 16384;  // compile-time bound of the LDS key buffer
-constexpr int SEL_ITEMS = LDS_KEYS_MAX / SEL_NT;
 constexpr int STREAM_NT = 256;       // threads of the streaming kernels
 // The streaming filter reads each key exactly once: non-temporal loads keep
 // it from evicting the row metadata and candidate buffers from the caches.
@@ -147,6 +158,12 @@ __global__ __launch_bounds__(256) void k_plan_rows(KParams p) {
     tiles = (uint32_t)((len + p.tune.tile - 1) / p.tune.tile);
     const int64_t cap = 4 * k + (len + 63) / 64 + 1024;
     s.cap = (uint32_t)(cap < len ? cap : len);
+    // A row whose slots hold all of it gains nothing from a threshold: every
+    // key becomes a candidate (T = 0) and the select works on the whole row.
+    if (s.phase == PH_SAMPLE && cap >= len) {
+      s.phase = PH_FILTER;
+      s.T = 0;
+    }
   }
   p.desc[r] = d;
   p.st[r] = s;
@@ -161,9 +178,13 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
   __shared__ uint32_t scratch[16];
   __shared__ uint64_t scratch64[16];
   __shared__ uint32_t bad_sh;
+  __shared__ uint32_t hist_sh;
   __shared__ uint32_t status_sh;
   const int tid = threadIdx.x;
-  if (tid == 0) bad_sh = 0;
+  if (tid == 0) {
+    bad_sh = 0;
+    hist_sh = 0;
+  }
   __syncthreads();
   uint32_t carry_t = 0;
   uint64_t carry_c = 0;
@@ -172,6 +193,7 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
     const uint32_t t = r < p.R ? p.tile_start[r] : 0u;
     const uint64_t c = r < p.R ? p.cand_base[r] : 0ull;
     if (r < p.R && (p.st[r].flags & 0x80000000u)) atomicOr(&bad_sh, 1u);
+    if (r < p.R && p.st[r].phase == PH_HIST) atomicAdd(&hist_sh, 1u);
     uint32_t tot_t;
     const uint32_t ex_t = block_excl_scan<1024>(t, scratch, tot_t);
     // 64-bit scan of the capacities: wave scan + per-wave totals.
@@ -205,8 +227,11 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
     Header* h = p.hdr;
     h->total_tiles = carry_t;
     h->total_cand = carry_c;
-    h->fallback_rows = 0;
+    h->fallback_rows = hist_sh;  // rows that start in the exact passes
     h->ordered_rows = 0;
+    h->q_sample = 0;
+    h->q_select[0] = 0;
+    h->q_select[1] = 0;
     if (carry_t > p.tile_capacity) status |= ST_TILES;
     if (carry_c > p.cand_capacity) status |= ST_CANDIDATES;
     h->status = status;
@@ -214,6 +239,31 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
     if (status & (ST_TILES | ST_CANDIDATES)) p.tile_start[p.R] = 0;  // disable streaming
   }
   __syncthreads();
+  // Row order of the per-row kernels: counting sort by floor(log2 k),
+  // largest first, so the longest selections and sorts start first.
+  {
+    __shared__ uint32_t bucket[16];
+    if (tid < 16) bucket[tid] = 0;
+    __syncthreads();
+    for (int r = tid; r < p.R; r += 1024) {
+      const uint32_t k = (uint32_t)max(p.desc[r].k, 1);
+      atomicAdd(&bucket[min(31 - __builtin_clz(k), 15)], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t acc = 0;
+      for (int c = 15; c >= 0; --c) {
+        const uint32_t n = bucket[c];
+        bucket[c] = acc;
+        acc += n;
+      }
+    }
+    __syncthreads();
+    for (int r = tid; r < p.R; r += 1024) {
+      const uint32_t k = (uint32_t)max(p.desc[r].k, 1);
+      p.order[atomicAdd(&bucket[min(31 - __builtin_clz(k), 15)], 1u)] = (uint32_t)r;
+    }
+  }
   // On a capacity overflow no long row is processed (the status reports it).
   if (status_sh & (ST_TILES | ST_CANDIDATES)) {
     for (int r = tid; r < p.R; r += 1024) {
@@ -240,32 +290,53 @@ __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
   __shared__ uint32_t scratch[16];
   __shared__ uint64_t scratch64[16];
   __shared__ uint32_t outv[4];
-  for (int r = blockIdx.x; r < p.R; r += gridDim.x) {
+  __shared__ uint32_t next_sh;
+  for (;;) {
+    if (threadIdx.x == 0) next_sh = atomicAdd(&p.hdr->q_sample, 1u);
+    __syncthreads();
+    const uint32_t qi = next_sh;
+    __syncthreads();
+    if (qi >= (uint32_t)p.R) break;
+    const int r = (int)p.order[qi];
     if (p.st[r].phase != PH_SAMPLE) continue;
+    RK_STAMP(p, r, 0);
     const RowDesc d = p.desc[r];
     const U* rb = static_cast<const U*>(d.base);
     const int64_t len = d.len;
     const int64_t ge = p.tune.group_elems;
     int64_t ng = p.tune.sample_groups;
-    const int64_t max_keys = (int64_t)p.lds_keys * 2;
+    const int64_t max_keys = p.sample_keys;
     if (ng * ge > max_keys) ng = max_keys / ge;
     if (ng * ge * 64 > len) ng = len / (64 * ge);  // read about 1/64 of the row,
     const int64_t ng_min = len / (2 * ge) < 64 ? len / (2 * ge) : 64;
     if (ng < ng_min) ng = ng_min;                   // but at least 64 runs
     if (ng < 1) ng = 1;
     const uint32_t s = (uint32_t)(ng * ge);
-    for (uint32_t i = threadIdx.x; i < s; i += SEL_NT) {
-      const int64_t g = i / ge, j = i % ge;
-      const int64_t start = ng > 1 ? (g * (len - ge)) / (ng - 1) : 0;
-      keys[i] = sel_key<DT, LARGEST>((uint32_t)rb[start + j]);
+    constexpr int UN = 8;  // loads in flight per thread
+    for (uint32_t i0 = threadIdx.x; i0 < s; i0 += SEL_NT * UN) {
+      uint32_t u[UN];
+#pragma unroll
+      for (int q = 0; q < UN; ++q) {
+        const uint32_t i = i0 + q * SEL_NT;
+        const int64_t g = i / ge, j = i % ge;
+        const int64_t start = ng > 1 ? (g * (len - ge)) / (ng - 1) : 0;
+        u[q] = i < s ? (uint32_t)ldg(rb + start + j) : 0u;
+      }
+#pragma unroll
+      for (int q = 0; q < UN; ++q) {
+        const uint32_t i = i0 + q * SEL_NT;
+        if (i < s) keys[i] = sel_key<DT, LARGEST>(u[q]);
+      }
     }
     __syncthreads();
+    RK_STAMP(p, r, 1);
     const float mu = (float)s * (float)d.k / (float)len;
     float rf = ceilf(mu + 4.0f * sqrtf(mu) + 2.0f);
     uint32_t rank = rf >= (float)s ? s : (uint32_t)rf;
     if (rank < 1) rank = 1;
     uint32_t T = 0;
-    if (rank < s) T = block_radix_select<SEL_NT, 11, uint32_t>(keys, s, rank, hist, scratch, scratch64, outv);
+    if (rank < s) T = block_radix_select<SEL_NT, 11, uint32_t, false>(keys, s, rank, hist, scratch, scratch64, outv);
+    RK_STAMP(p, r, 2);
     uint32_t ge_cnt = 0;
     for (uint32_t i = threadIdx.x; i < s; i += SEL_NT) ge_cnt += (keys[i] >= T) ? 1u : 0u;
     ge_cnt = block_sum<SEL_NT>(ge_cnt, scratch);
@@ -354,6 +425,7 @@ RK_DEV void append_vec(const AppendCtx& ac, uint32_t& n, uint4 x, uint32_t mask,
 
 template <int DT, bool LARGEST, int U>
 __global__ __launch_bounds__(STREAM_NT) void k_filter(KParams p) {
+  if (p.round && p.hdr->fallback_rows == 0) return;  // nothing fell back
   using UT = typename Traits<DT>::U;
   constexpr int P = Traits<DT>::PACK;
   __shared__ uint64_t stage[STREAM_NT / 64][WSTAGE];
@@ -521,7 +593,7 @@ RK_DEV void sort_and_write(const KParams& p, const RowDesc& d, uint64_t* sbuf, u
   for (uint32_t i = threadIdx.x; i < m; i += SEL_NT) {
     uint32_t sk, pos;
     order_decode<DT>(mode, sbuf[i], sk, pos);
-    const uint32_t bits = sk ? key_to_bits<DT, LARGEST>(sk) : (uint32_t)rb[pos];
+    const uint32_t bits = sk ? key_to_bits<DT, LARGEST>(sk) : (uint32_t)ldg(rb + pos);
     ov[i] = (U)bits;
     oi[i] = d.idx_in ? d.idx_in[pos] : (int32_t)pos;
   }
@@ -541,6 +613,7 @@ RK_DEV void sort_and_write(const KParams& p, const RowDesc& d, uint64_t* sbuf, u
 // ---------------------------------------------------------------------------
 template <int DT, bool LARGEST>
 __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
+  if (p.round && p.hdr->fallback_rows == 0) return;
   using U = typename Traits<DT>::U;
   extern __shared__ uint64_t sbuf[];
   __shared__ uint32_t hist[HIST_MAX_BINS];
@@ -548,10 +621,18 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
   __shared__ uint64_t scratch64[16];
   __shared__ uint32_t outv[4];
   __shared__ uint32_t slot_cnt;
+  __shared__ uint32_t next_sh;
   const int mode = order_mode(p.flags);
-  for (int r = blockIdx.x; r < p.R; r += gridDim.x) {
+  for (;;) {
+    if (threadIdx.x == 0) next_sh = atomicAdd(&p.hdr->q_select[p.round], 1u);
+    __syncthreads();
+    const uint32_t qi = next_sh;
+    __syncthreads();
+    if (qi >= (uint32_t)p.R) break;
+    const int r = (int)p.order[qi];
     const uint32_t ph = p.st[r].phase;
     if (ph != PH_COPYALL && ph != PH_SMALL && ph != PH_FILTER) continue;
+    RK_STAMP(p, r, 8);
     const RowDesc d = p.desc[r];
     const U* rb = static_cast<const U*>(d.base);
     const int64_t len = d.len;
@@ -559,7 +640,7 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
     if (ph == PH_COPYALL) {
       const uint32_t m = (uint32_t)(len < (int64_t)k ? len : (int64_t)k);
       for (uint32_t i = threadIdx.x; i < m; i += SEL_NT)
-        sbuf[i] = order_key<DT>(mode, sel_key<DT, LARGEST>((uint32_t)rb[i]), i);
+        sbuf[i] = order_key<DT>(mode, sel_key<DT, LARGEST>((uint32_t)ldg(rb + i)), i);
       __syncthreads();
       sort_and_write<DT, LARGEST>(p, d, sbuf, m, mode);
     } else {
@@ -569,7 +650,7 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
       if (ph == PH_SMALL) {
         n = (uint32_t)len;
         for (uint32_t i = threadIdx.x; i < n; i += SEL_NT)
-          sbuf[i] = ((uint64_t)sel_key<DT, LARGEST>((uint32_t)rb[i]) << 32) | (uint32_t)(~i);
+          sbuf[i] = ((uint64_t)sel_key<DT, LARGEST>((uint32_t)ldg(rb + i)) << 32) | (uint32_t)(~i);
         keys = sbuf;
         in_lds = true;
       } else {
@@ -581,14 +662,17 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
         if (ok) {
           in_lds = n <= (uint32_t)p.lds_keys;
           if (in_lds) {
-            for (uint32_t i = threadIdx.x; i < n; i += SEL_NT) sbuf[i] = src[i];
+            for (uint32_t i = threadIdx.x; i < n; i += SEL_NT) sbuf[i] = ldg(src + i);
             keys = sbuf;
           } else {
             keys = src;
           }
           __syncthreads();
           uint32_t ge = 0;
-          for (uint32_t i = threadIdx.x; i < n; i += SEL_NT) ge += ((uint32_t)(keys[i] >> 32) >= st.T) ? 1u : 0u;
+          for (uint32_t i = threadIdx.x; i < n; i += SEL_NT) {
+            const uint64_t x = in_lds ? ld_lds(keys + i) : ldg(keys + i);
+            ge += ((uint32_t)(x >> 32) >= st.T) ? 1u : 0u;
+          }
           ge = block_sum<SEL_NT>(ge, scratch);
           ok = ge >= k;
         }
@@ -605,28 +689,38 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
         }
       }
       __syncthreads();
-      const uint64_t kth = block_radix_select<SEL_NT, 11, uint64_t>(keys, n, k, hist, scratch, scratch64, outv);
+      RK_STAMP(p, r, 9);
+      const uint64_t kth =
+          in_lds ? block_radix_select<SEL_NT, 11, uint64_t, false>(keys, n, k, hist, scratch, scratch64, outv)
+                 : block_radix_select<SEL_NT, 11, uint64_t, true>(keys, n, k, hist, scratch, scratch64, outv);
+      RK_STAMP(p, r, 10);
       if (threadIdx.x == 0) slot_cnt = 0;
       if (in_lds) {
-        uint64_t reg[SEL_ITEMS];
+        // In place, one chunk of CH keys per thread at a time: every slot
+        // handed out is below the number of keys read so far, so a write
+        // never lands on a key that is still unread.
+        constexpr int CH = 4;
+        for (uint32_t c0 = 0; c0 < n; c0 += SEL_NT * CH) {
+          uint64_t reg[CH];
 #pragma unroll
-        for (int j = 0; j < SEL_ITEMS; ++j) {
-          const uint32_t i = threadIdx.x + j * SEL_NT;
-          reg[j] = i < n ? sbuf[i] : 0ull;
-        }
-        __syncthreads();
+          for (int j = 0; j < CH; ++j) {
+            const uint32_t i = c0 + threadIdx.x + j * SEL_NT;
+            reg[j] = i < n ? sbuf[i] : 0ull;
+          }
+          __syncthreads();
 #pragma unroll
-        for (int j = 0; j < SEL_ITEMS; ++j) {
-          const uint32_t i = threadIdx.x + j * SEL_NT;
-          if (i < n && reg[j] >= kth) {
-            const uint32_t slot = atomicAdd(&slot_cnt, 1u);
-            if (slot < k) sbuf[slot] = order_key<DT>(mode, (uint32_t)(reg[j] >> 32), ~(uint32_t)reg[j]);
+          for (int j = 0; j < CH; ++j) {
+            const uint32_t i = c0 + threadIdx.x + j * SEL_NT;
+            if (i < n && reg[j] >= kth) {
+              const uint32_t slot = atomicAdd(&slot_cnt, 1u);
+              if (slot < k) sbuf[slot] = order_key<DT>(mode, (uint32_t)(reg[j] >> 32), ~(uint32_t)reg[j]);
+            }
           }
         }
       } else {
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < n; i += SEL_NT) {
-          const uint64_t x = keys[i];
+          const uint64_t x = ldg(keys + i);
           if (x >= kth) {
             const uint32_t slot = atomicAdd(&slot_cnt, 1u);
             if (slot < k) sbuf[slot] = order_key<DT>(mode, (uint32_t)(x >> 32), ~(uint32_t)x);
@@ -635,7 +729,9 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
       }
       __syncthreads();
       if (threadIdx.x == 0 && slot_cnt != k) atomicOr(&p.hdr->status, (uint32_t)ST_INTERNAL);
+      RK_STAMP(p, r, 11);
       sort_and_write<DT, LARGEST>(p, d, sbuf, k, mode);
+      RK_STAMP(p, r, 12);
     }
     __syncthreads();
     if (threadIdx.x == 0) p.st[r].phase = PH_DONE;
@@ -652,6 +748,7 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
 // resolved, as PH_ORDERED.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_exact_init(KParams p) {
+  if (p.hdr->fallback_rows == 0) return;
   for (int r = blockIdx.x; r < p.R; r += gridDim.x) {
     const RowState s = p.st[r];
     if (s.phase != PH_HIST || !(s.flags & RF_NEED_INIT)) continue;
@@ -670,6 +767,7 @@ __global__ __launch_bounds__(256) void k_exact_init(KParams p) {
 
 template <int DT, bool LARGEST, int U>
 __global__ __launch_bounds__(STREAM_NT) void k_hist(KParams p) {
+  if (p.hdr->fallback_rows == 0) return;
   using UT = typename Traits<DT>::U;
   constexpr int P = Traits<DT>::PACK;
   constexpr int KB = Traits<DT>::KEY_BITS;
@@ -756,6 +854,7 @@ __global__ __launch_bounds__(STREAM_NT) void k_hist(KParams p) {
 }
 
 __global__ __launch_bounds__(256) void k_hist_select(KParams p, int key_bits) {
+  if (p.hdr->fallback_rows == 0) return;
   __shared__ uint32_t hist[HIST_MAX_BINS];
   __shared__ uint32_t scratch[16];
   __shared__ uint32_t outv[4];
@@ -844,6 +943,7 @@ RK_DEV uint32_t load_slot(const typename Traits<DT>::U* rb, int64_t es, int64_t 
 
 template <int DT, bool LARGEST>
 __global__ __launch_bounds__(STREAM_NT) void k_emit_count(KParams p) {
+  if (p.hdr->ordered_rows == 0) return;
   using UT = typename Traits<DT>::U;
   constexpr int P = Traits<DT>::PACK;
   __shared__ uint32_t scratch[16];
@@ -881,6 +981,7 @@ __global__ __launch_bounds__(STREAM_NT) void k_emit_count(KParams p) {
 }
 
 __global__ __launch_bounds__(1024) void k_emit_scan(KParams p) {
+  if (p.hdr->ordered_rows == 0) return;
   __shared__ uint32_t scratch[16];
   for (int r = blockIdx.x; r < p.R; r += gridDim.x) {
     if (p.st[r].phase != PH_ORDERED) continue;
@@ -901,6 +1002,7 @@ __global__ __launch_bounds__(1024) void k_emit_scan(KParams p) {
 
 template <int DT, bool LARGEST>
 __global__ __launch_bounds__(STREAM_NT) void k_emit_write(KParams p) {
+  if (p.hdr->ordered_rows == 0) return;
   using UT = typename Traits<DT>::U;
   constexpr int P = Traits<DT>::PACK;
   __shared__ uint32_t scratch[16];
@@ -972,6 +1074,7 @@ __global__ __launch_bounds__(STREAM_NT) void k_emit_write(KParams p) {
 
 template <int DT, bool LARGEST>
 __global__ __launch_bounds__(SEL_NT) void k_finalize_ordered(KParams p) {
+  if (p.hdr->ordered_rows == 0) return;
   using U = typename Traits<DT>::U;
   extern __shared__ uint64_t sbuf[];
   const int mode = order_mode(p.flags);
@@ -1049,17 +1152,17 @@ static void timing_end(hipStream_t s, const std::pair<hipEvent_t, hipEvent_t>& e
 // Workgroups of a persistent kernel that can be resident at once on the
 // device: the static partition needs every workgroup running from the start.
 template <typename K>
-static int resident_grid(K kernel, int threads) {
+static int resident_grid(K kernel, int threads, size_t dyn = 0) {
   static std::mutex mu;
-  static std::vector<std::pair<const void*, int>> memo;
-  const void* key = reinterpret_cast<const void*>(kernel);
+  static std::vector<std::pair<std::pair<const void*, size_t>, int>> memo;
+  const std::pair<const void*, size_t> key{reinterpret_cast<const void*>(kernel), dyn};
   std::lock_guard<std::mutex> lk(mu);
   for (auto& m : memo)
     if (m.first == key) return m.second;
   int dev = 0, cus = 0, per_cu = 0;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, dyn) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
   const int g = cus * per_cu;
@@ -1071,33 +1174,46 @@ template <int DT, bool LARGEST, int U>
 static hipError_t run_typed_u(const KParams& p, const Launch& L0) {
   constexpr int KB = Traits<DT>::KEY_BITS;
   const size_t dyn = (size_t)p.lds_keys * sizeof(uint64_t);
+  const size_t dyn_sample = (size_t)p.sample_keys * sizeof(uint32_t);
   const int rows_blocks = (p.R + 255) / 256;
   hipStream_t s = L0.stream;
-  Launch L = L0;
-  L.grid = std::min(L0.grid, resident_grid(k_filter<DT, LARGEST, U>, STREAM_NT));
-  const int hist_grid = std::min(L0.grid, resident_grid(k_hist<DT, LARGEST, U>, STREAM_NT));
+  // Persistent streaming kernels: every workgroup resident from the start.
+  const int fgrid = std::min(L0.grid, resident_grid(k_filter<DT, LARGEST, U>, STREAM_NT));
+  const int hgrid = std::min(L0.grid, resident_grid(k_hist<DT, LARGEST, U>, STREAM_NT));
+  // Per-row kernels loop over rows: one resident wave of workgroups suffices.
+  auto rows = [&](int resident) { return std::max(1, std::min(L0.row_grid, resident)); };
+  const int g_sample = rows(resident_grid(k_sample<DT, LARGEST>, SEL_NT, dyn_sample));
+  const int g_select = rows(resident_grid(k_select_finalize<DT, LARGEST>, SEL_NT, dyn));
+  const int g_init = rows(resident_grid(k_exact_init, 256));
+  const int g_hsel = rows(resident_grid(k_hist_select, 256));
+  const int g_escan = rows(resident_grid(k_emit_scan, 1024));
+  const int g_ford = rows(resident_grid(k_finalize_ordered<DT, LARGEST>, SEL_NT, dyn));
+  KParams p1 = p;  // fallback round: kernels return at once when no row fell back
+  p1.round = 1;
   k_plan_rows<DT><<<rows_blocks, 256, 0, s>>>(p);
   k_plan_scan<<<1, 1024, 0, s>>>(p);
   if (!(p.flags & FL_EXACT)) {
-    k_sample<DT, LARGEST><<<L.row_grid, SEL_NT, dyn, s>>>(p);
+    k_sample<DT, LARGEST><<<g_sample, SEL_NT, dyn_sample, s>>>(p);
     std::pair<hipEvent_t, hipEvent_t> ev;
     const bool timed = timing_begin(s, ev);
-    k_filter<DT, LARGEST, U><<<L.grid, STREAM_NT, 0, s>>>(p);
+    k_filter<DT, LARGEST, U><<<fgrid, STREAM_NT, 0, s>>>(p);
     if (timed) timing_end(s, ev);
-    k_select_finalize<DT, LARGEST><<<L.row_grid, SEL_NT, dyn, s>>>(p);
+    k_select_finalize<DT, LARGEST><<<g_select, SEL_NT, dyn, s>>>(p);
   }
-  k_exact_init<<<L.row_grid, 256, 0, s>>>(p);
+  k_exact_init<<<g_init, 256, 0, s>>>(p1);
   const int passes = (KB + p.tune.hist_bits - 1) / p.tune.hist_bits;
   for (int i = 0; i < passes; ++i) {
-    k_hist<DT, LARGEST, U><<<hist_grid, STREAM_NT, 0, s>>>(p);
-    k_hist_select<<<L.row_grid, 256, 0, s>>>(p, KB);
+    k_hist<DT, LARGEST, U><<<hgrid, STREAM_NT, 0, s>>>(p1);
+    k_hist_select<<<g_hsel, 256, 0, s>>>(p1, KB);
   }
-  k_filter<DT, LARGEST, U><<<L.grid, STREAM_NT, 0, s>>>(p);
-  k_select_finalize<DT, LARGEST><<<L.row_grid, SEL_NT, dyn, s>>>(p);
-  k_emit_count<DT, LARGEST><<<L.grid, STREAM_NT, 0, s>>>(p);
-  k_emit_scan<<<L.row_grid, 1024, 0, s>>>(p);
-  k_emit_write<DT, LARGEST><<<L.grid, STREAM_NT, 0, s>>>(p);
-  k_finalize_ordered<DT, LARGEST><<<L.row_grid, SEL_NT, dyn, s>>>(p);
+  k_filter<DT, LARGEST, U><<<fgrid, STREAM_NT, 0, s>>>(p1);
+  // COPYALL / SMALL rows of an exact-path call are finalized here too.
+  KParams pf = (p.flags & FL_EXACT) ? p : p1;
+  k_select_finalize<DT, LARGEST><<<g_select, SEL_NT, dyn, s>>>(pf);
+  k_emit_count<DT, LARGEST><<<fgrid, STREAM_NT, 0, s>>>(p1);
+  k_emit_scan<<<g_escan, 1024, 0, s>>>(p1);
+  k_emit_write<DT, LARGEST><<<fgrid, STREAM_NT, 0, s>>>(p1);
+  k_finalize_ordered<DT, LARGEST><<<g_ford, SEL_NT, dyn, s>>>(p1);
   return hipGetLastError();
 }
 

@@ -68,8 +68,11 @@ struct Header {
   uint64_t total_cand;
   uint32_t fallback_rows;  // rows that left the sampled path for exact passes
   uint32_t ordered_rows;   // rows that needed the ordered emit
-  uint32_t pad[58];
+  uint32_t q_sample;       // row queue heads of the per-row kernels
+  uint32_t q_select[2];    //   (select: sampled round, fallback round)
+  uint32_t pad[55];
 };
+static_assert(sizeof(Header) == 256, "workspace header is 256 bytes");
 
 // Tunables chosen by the autotuner per shape bucket.
 struct Tune {
@@ -106,14 +109,17 @@ struct KParams {
   uint32_t* tile_start;  // [R + 1]
   uint64_t* cand_base;   // [R + 1]
   uint32_t* cand_cnt;    // [R]
+  uint32_t* order;       // [R] rows by descending cost class (largest k first)
   uint32_t* ghist;       // [R][2048]
   uint2* tile_cnt;       // [tile_capacity]
   uint2* tile_pre;       // [tile_capacity]
   uint64_t* cand;        // [cand_capacity]
   uint64_t tile_capacity;
   uint64_t cand_capacity;
-  int32_t lds_keys;  // LDS key capacity of the select kernels
-  int32_t small_max; // rows up to this length select straight from the input
+  int32_t lds_keys;     // LDS key capacity of the select kernels (this launch)
+  int32_t small_max;    // rows up to this length select straight from the input
+  int32_t sample_keys;  // LDS key capacity of the sample kernel (this launch)
+  int32_t round;        // 0: sampled round; 1: fallback round (skips when idle)
   Tune tune;
 };
 
