@@ -36,6 +36,20 @@ def log(msg: str) -> None:
     sys.stderr.flush()
 
 
+def profiled_traffic(workload: str):
+    """HBM bytes per k_filter launch measured by the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes summarized in profiles/k_filter_<workload>.txt (scaled
+    by known-size kernels), or None when no summary exists."""
+    path = os.path.join(ROOT, "profiles", f"k_filter_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
 def make_workload(name: str):
     import radik_amd.workloads as W
     table = {
@@ -161,8 +175,14 @@ def main() -> int:
 
     # Dominant kernel: the streaming filter.  Per launch it reads every key of
     # the rows longer than the LDS-direct limit once: sizeof(T) bytes per key.
-    small_max = rk.max_k()
-    long_keys = int(w.lens[w.lens > small_max].sum())
+    # Rows the filter streams: longer than the LDS-direct limit of this call
+    # (next power of two >= max(4096, max_k), as the library sizes it) and
+    # longer than their k (shorter rows are copied whole).
+    small_max = 4096
+    while small_max < w.max_k:
+        small_max *= 2
+    streamed = (w.lens > small_max) & (w.lens > w.ks)
+    long_keys = int(w.lens[streamed].sum())
     filt_bytes = long_keys * es
     filt_avg_ms = filt_ms / max(filt_n, 1)
     achieved = filt_bytes / (filt_avg_ms / 1e3) / 1e9
@@ -187,7 +207,8 @@ def main() -> int:
                    "largest": True, "sorted": True, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "k_filter", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes_per_launch": filt_bytes,
+                     "traffic": profiled_traffic(args.workload),
+                     "algorithmic_bytes_per_launch": filt_bytes,
                      "avg_launch_ms": filt_avg_ms, "launches_timed": filt_n},
         "pipeline": {"algorithmic_bytes": alg_bytes, "achieved_gbs": pipeline_gbs,
                      "roofline_frac": pipeline_gbs / HBM_PEAK_GBS,
