@@ -12,6 +12,10 @@ Prints one JSON line with:
   cpu_baseline the multithreaded CPU oracle (std::nth_element + std::sort)
                on a bounded sample of the same rows, on this host's cores
   plain        the same rows with one RadiK multi-pass call per row
+  serving      the serving loop on the mixed request stream (100k requests,
+               Poisson arrivals): p50/p99 latency and throughput of the
+               continuous-batching server and of per-request plain calls at
+               the same offered rate
 Progress lines go to stderr.
 """
 
@@ -73,6 +77,12 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample-keys", type=float, default=1.0e9,
                     help="bound on the keys in the CPU baseline's sample")
+    ap.add_argument("--serve-requests", type=int, default=100_000,
+                    help="requests in the serving run (0: skip it)")
+    ap.add_argument("--serve-load", type=float, default=0.7,
+                    help="offered rate as a fraction of the plain server's capacity")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the serving measurement (prints its JSON object)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -89,6 +99,9 @@ def main() -> int:
     import radik_amd.workloads as W
 
     torch.cuda.set_device(0)
+    if args.serve_only:
+        print(json.dumps({"serving": serving(args)}), flush=True)
+        return 0
     w = make_workload(args.workload)
     es = 4 if w.dtype == "f32" else 2
     dt = {"f32": rk.RK_F32, "bf16": rk.RK_BF16, "f16": rk.RK_F16}[w.dtype]
@@ -288,8 +301,108 @@ def main() -> int:
                       f"{nth} threads, {c_s:.2f} s"}
         log(f"cpu baseline: {float(s_off[-1]) / c_s / 1e9:.3f} Gkeys/s on {nth} threads")
 
+    if args.serve_requests > 0:
+        del vals, ws, out_v, out_i
+        if not args.no_plain and args.plain_steps > 0:
+            del pws
+        torch.cuda.empty_cache()
+        result["serving"] = serving(args)
+
     print(json.dumps(result), flush=True)
     return 0
+
+
+def percentile(x, q):
+    """Nearest-rank percentile (the server's own statistic)."""
+    import numpy as np
+    x = np.sort(np.asarray(x))
+    return float(x[max(1, int(np.ceil(q * len(x)))) - 1]) if len(x) else 0.0
+
+
+def serving(args) -> dict:
+    """Serving-loop measurement.  Request stream: n requests with length
+    floor(2^U(12,22)), k floor(2^U(0,12)) and one of four value families
+    (uniform, normal, Zipf s=1.1, near-constant with 30% duplicates); each
+    request reads a prefix of a resident 2^22-element body of its family
+    (256 bodies per family, 16 GB).  Poisson arrivals at `load` times the
+    plain server's measured capacity; the continuous-batching server and the
+    plain server (one RK_EXACT call per request, one stream) replay the same
+    schedule against the same 32 GB admission budget."""
+    import numpy as np
+    import torch
+
+    import radik_amd as rk
+    import radik_amd.workloads as W
+
+    n = args.serve_requests
+    lens, ks, kinds = W.c5_requests(n)
+    per_kind, body_len = 256, 1 << 22
+    bodies = W.c5_bodies(per_kind=per_kind, body_len=body_len)
+    body = kinds.astype(np.int64) * per_kind + (np.arange(n) % per_kind)
+    vptr = bodies.data_ptr() + body * body_len * 4
+    oo = np.zeros(n + 1, np.int64)
+    oo[1:] = np.cumsum(ks.astype(np.int64))
+    ov = torch.empty(int(oo[-1]), dtype=torch.float32, device="cuda")
+    oi = torch.empty(int(oo[-1]), dtype=torch.int32, device="cuda")
+    optr_v = ov.data_ptr() + oo[:-1] * 4
+    optr_i = oi.data_ptr() + oo[:-1] * 4
+    torch.cuda.synchronize()
+    log(f"serving: {n} requests, mean len {lens.mean():.0f}, mean k {ks.mean():.0f}, "
+        f"bodies {bodies.numel() * 4 / 1e9:.1f} GB resident")
+    budget = 32 << 30
+    common = dict(max_k=4096, mem_budget=budget, max_batch_keys=1 << 27)
+    batched_cfg = dict(max_batch_rows=1024, streams=4, **common)
+
+    def run(plain, idx, arrivals):
+        cfg = dict(plain=True, **common) if plain else batched_cfg
+        with rk.Server(**cfg) as srv:
+            lat, makespan = srv.replay(vptr[idx], lens[idx], ks[idx], optr_v[idx], optr_i[idx],
+                                       arrivals)
+            st = srv.stats()
+        if st["failed"] or st["rejected"] or np.any(lat < 0):
+            raise RuntimeError(f"serving run failed: {st}")
+        return lat, makespan, st
+
+    # warm both paths (module load, LDS attributes, resident-grid queries)
+    warm = np.arange(min(n, 256))
+    run(True, warm, np.zeros(len(warm)))
+    run(False, warm, np.zeros(len(warm)))
+    # capacity: every request of a prefix submitted at t = 0 (closed burst)
+    n_cap = min(n, 4000)
+    cap_idx = np.arange(n_cap)
+    _, mk_plain, _ = run(True, cap_idx, np.zeros(n_cap))
+    cap_plain = n_cap / mk_plain
+    n_cap_b = min(n, 40000)
+    _, mk_b, st_cap = run(False, np.arange(n_cap_b), np.zeros(n_cap_b))
+    cap_batched = n_cap_b / mk_b
+    log(f"capacity: plain {cap_plain:.0f} req/s, batched {cap_batched:.0f} req/s "
+        f"(mean batch {st_cap['mean_batch_rows']:.1f} rows)")
+    rate = args.serve_load * cap_plain
+    arrivals = W.poisson_arrivals(n, rate)
+    idx = np.arange(n)
+    out = {"requests": n, "offered_rate_per_s": rate, "load_vs_plain_capacity": args.serve_load,
+           "mem_budget_bytes": budget,
+           "workload": "len floor(2^U(12,22)), k floor(2^U(0,12)), fp32 mixture of uniform / "
+                       "normal / Zipf(1.1) / near-constant+30% duplicates, Poisson arrivals",
+           "latency": "scheduled arrival to observed completion, us"}
+    for name, plain in (("batched", False), ("plain", True)):
+        lat, makespan, st = run(plain, idx, arrivals)
+        out[name] = {"p50_us": percentile(lat, 0.50), "p99_us": percentile(lat, 0.99),
+                     "mean_us": float(lat.mean()), "max_us": float(lat.max()),
+                     "throughput_req_s": n / makespan,
+                     "keys_per_s": float(lens.sum()) / makespan,
+                     "batches": st["batches"], "mean_batch_rows": st["mean_batch_rows"],
+                     "fallback_batches": st["fallback_batches"],
+                     "peak_inflight_bytes": st["peak_inflight_bytes"],
+                     "server_bytes": st["server_bytes"]}
+        log(f"{name}: p50 {out[name]['p50_us']:.0f} us, p99 {out[name]['p99_us']:.0f} us, "
+            f"{out[name]['throughput_req_s']:.0f} req/s, mean batch {st['mean_batch_rows']:.2f}")
+    out["batched"]["capacity_req_s"] = cap_batched
+    out["plain"]["capacity_req_s"] = cap_plain
+    out["p99_ratio_plain_over_batched"] = out["plain"]["p99_us"] / out["batched"]["p99_us"]
+    del bodies, ov, oi
+    torch.cuda.empty_cache()
+    return out
 
 
 if __name__ == "__main__":

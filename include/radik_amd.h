@@ -166,6 +166,66 @@ int rk_graph_end(void* stream, rk_graph** out);
 int rk_graph_launch(rk_graph* graph, void* stream);
 int rk_graph_destroy(rk_graph* graph);
 
+/* ---------------------------------------------------------------------------
+ * Serving loop (no counterpart in the reference, whose harness launches one
+ * batch per call, radik/test/test_radik.cu:135-160).  A server owns `streams`
+ * slots, each a stream with its own workspace, sized at creation for
+ * max_batch_rows rows of at most max_batch_keys keys in total; creation is
+ * the only call that allocates device memory.  A dispatcher thread admits
+ * pending requests in arrival order while the bytes of the requests in
+ * flight (input + output) plus the server's own buffers stay within
+ * mem_budget, coalesces them into one rows-form call per free slot, and
+ * completes them when the slot's event fires.  A request whose bytes alone
+ * exceed the budget is rejected (RK_E_WORKSPACE from rk_server_wait).
+ * plain = 1 is the baseline: one request per RK_EXACT call, one stream.
+ * Results are identical to rk_topk_rows on the request alone.
+ * ------------------------------------------------------------------------- */
+typedef struct rk_server rk_server;
+
+typedef struct rk_server_config {
+  int64_t max_batch_keys; /* keys per batch; also the longest request */
+  int64_t mem_budget;     /* device bytes: server buffers + requests in flight */
+  int32_t dtype;
+  uint32_t flags;         /* RK_LARGEST / RK_SORTED / RK_ASCEND of every request */
+  int32_t max_k;          /* largest k of any request */
+  int32_t max_batch_rows; /* requests per batch */
+  int32_t streams;        /* batches in flight at once */
+  int32_t plain;
+} rk_server_config;
+
+typedef struct rk_server_stats {
+  int64_t completed, rejected, failed;
+  int64_t batches, fallback_batches; /* fallback: batches that ran exact passes */
+  int64_t server_bytes, peak_inflight_bytes;
+  double mean_batch_rows;
+  double p50_us, p99_us, mean_us, max_us; /* latency: arrival to observed completion */
+  int32_t max_batch_rows;
+  int32_t pad_;
+} rk_server_stats;
+
+int rk_server_create(const rk_server_config* cfg, rk_server** out);
+/* Queues one request: a device row of `len` elements, k <= cfg.max_k; writes
+ * min(len, k) results and the tail fill up to k slots into out_vals /
+ * out_idx.  The row and outputs must stay valid until the request completes. */
+int rk_server_submit(rk_server* s, const void* vals, int64_t len, int32_t k, void* out_vals,
+                     int32_t* out_idx, uint64_t* ticket);
+/* Blocks until the request completes; returns its status and latency. */
+int rk_server_wait(rk_server* s, uint64_t ticket, double* latency_us);
+/* Blocks until nothing is queued or in flight. */
+int rk_server_drain(rk_server* s);
+/* Open-loop load: submits request i at arrival_s[i] seconds after the call
+ * starts (from the calling thread), waits for all, and returns per-request
+ * latency from the scheduled arrival (-1 if not completed) and the makespan.
+ * Returns RK_E_DEVICE if a request failed device-side validation; rejected
+ * requests only show as -1. */
+int rk_server_replay(rk_server* s, int32_t n, const void* const* vals, const int64_t* lens,
+                     const int32_t* ks, void* const* out_vals, int32_t* const* out_idx,
+                     const double* arrival_s, double* latency_us, double* makespan_s);
+/* Statistics over every request since creation (nearest-rank percentiles). */
+int rk_server_get_stats(rk_server* s, rk_server_stats* out);
+/* Drains, stops the dispatcher and frees the server's buffers. */
+int rk_server_destroy(rk_server* s);
+
 #ifdef __cplusplus
 }
 #endif

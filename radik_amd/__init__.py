@@ -21,17 +21,21 @@ from __future__ import annotations
 import ctypes
 import os
 
+import numpy as np
+
 __all__ = [
     "RK_F32", "RK_F16", "RK_BF16", "RK_LARGEST", "RK_SORTED", "RK_ASCEND", "RK_EXACT",
     "RadikError", "lib", "lib_path", "max_k", "get_workspace_size", "workspace_size",
     "topk_radix_select", "topk_batched", "topk_rows", "status", "gen_fill",
     "get_tune", "set_tune", "timing_enable", "timing_read", "autotune_batched",
-    "tune_cache_save", "tune_cache_load", "tune_cache_clear", "Graph",
+    "tune_cache_save", "tune_cache_load", "tune_cache_clear", "Graph", "Server",
 ]
 
 RK_F32, RK_F16, RK_BF16 = 0, 1, 2
 RK_LARGEST, RK_SORTED, RK_ASCEND, RK_EXACT = 1, 2, 4, 8
 GEN_UNIFORM, GEN_NORMAL, GEN_ZIPF, GEN_NEARCONST, GEN_RANGE, GEN_CONST = range(6)
+RK_OK, RK_E_ARG, RK_E_WORKSPACE, RK_E_UNSUPPORTED, RK_E_ALIGN, RK_E_HIP, RK_E_DEVICE = (
+    0, -1, -2, -3, -4, -5, -6)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.environ.get("RADIK_AMD_LIB") or os.path.join(_HERE, "lib", "libradik_amd.so")
@@ -50,7 +54,9 @@ EXPORTED = (
     "rk_get_tune", "rk_set_tune", "rk_gen_fill", "rk_timing_enable", "rk_timing_read",
     "rk_autotune_batched", "rk_tune_cache_save", "rk_tune_cache_load", "rk_tune_cache_clear",
     "rk_tune_cache_size", "rk_graph_begin", "rk_graph_end", "rk_graph_launch",
-    "rk_graph_destroy", "rk_probe_read",
+    "rk_graph_destroy", "rk_probe_read", "rk_server_create", "rk_server_submit",
+    "rk_server_wait", "rk_server_drain", "rk_server_replay", "rk_server_get_stats",
+    "rk_server_destroy",
 )
 
 
@@ -65,6 +71,22 @@ class RadikError(RuntimeError):
 class _Tune(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in
                 ("tile", "grid", "sample_groups", "group_elems", "hist_bits", "unroll")]
+
+
+class _ServerConfig(ctypes.Structure):
+    _fields_ = [("max_batch_keys", ctypes.c_int64), ("mem_budget", ctypes.c_int64),
+                ("dtype", ctypes.c_int32), ("flags", ctypes.c_uint32),
+                ("max_k", ctypes.c_int32), ("max_batch_rows", ctypes.c_int32),
+                ("streams", ctypes.c_int32), ("plain", ctypes.c_int32)]
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = ([(n, ctypes.c_int64) for n in
+                 ("completed", "rejected", "failed", "batches", "fallback_batches",
+                  "server_bytes", "peak_inflight_bytes")]
+                + [(n, ctypes.c_double) for n in
+                   ("mean_batch_rows", "p50_us", "p99_us", "mean_us", "max_us")]
+                + [("max_batch_rows", ctypes.c_int32), ("pad_", ctypes.c_int32)])
 
 
 def lib_path() -> str:
@@ -110,6 +132,17 @@ def lib():
         getattr(L, name).restype = c.c_int
     L.rk_probe_read.argtypes = [vp, i64, vp, vp]
     L.rk_probe_read.restype = c.c_int
+    P = c.POINTER
+    L.rk_server_create.argtypes = [P(_ServerConfig), P(vp)]
+    L.rk_server_submit.argtypes = [vp, vp, i64, i32, vp, vp, P(c.c_uint64)]
+    L.rk_server_wait.argtypes = [vp, c.c_uint64, P(c.c_double)]
+    L.rk_server_drain.argtypes = [vp]
+    L.rk_server_replay.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp, P(c.c_double)]
+    L.rk_server_get_stats.argtypes = [vp, P(_ServerStats)]
+    L.rk_server_destroy.argtypes = [vp]
+    for name in ("rk_server_create", "rk_server_submit", "rk_server_wait", "rk_server_drain",
+                 "rk_server_replay", "rk_server_get_stats", "rk_server_destroy"):
+        getattr(L, name).restype = c.c_int
     L.rk_timing_enable.argtypes = [c.c_int]
     L.rk_timing_read.argtypes = [c.POINTER(c.c_double), c.POINTER(i32)]
     for name in ("rk_timing_enable", "rk_timing_read",
@@ -386,3 +419,84 @@ def get_tune() -> dict:
 def set_tune(**kw) -> None:
     t = _Tune(**{n: int(kw.get(n, 0)) for n, _ in _Tune._fields_})
     _check(lib().rk_set_tune(ctypes.byref(t)), "rk_set_tune")
+
+
+class Server:
+    """The continuous-batching serving loop (rk_server_*).
+
+    Requests are device rows (tensors, or raw device pointers with lengths)
+    and their own output buffers; ``replay`` drives an open-loop arrival
+    schedule from the C++ side and returns per-request latencies (us).
+    """
+
+    def __init__(self, dtype=RK_F32, *, largest=True, sorted=True, ascend=False, max_k=4096,
+                 max_batch_rows=1024, max_batch_keys=1 << 26, streams=4,
+                 mem_budget=32 << 30, plain=False):
+        flags = ((RK_LARGEST if largest else 0) | (RK_SORTED if sorted else 0)
+                 | (RK_ASCEND if ascend else 0))
+        cfg = _ServerConfig(max_batch_keys=int(max_batch_keys), mem_budget=int(mem_budget),
+                            dtype=int(dtype), flags=flags, max_k=int(max_k),
+                            max_batch_rows=int(max_batch_rows), streams=int(streams),
+                            plain=1 if plain else 0)
+        h = ctypes.c_void_p()
+        _check(lib().rk_server_create(ctypes.byref(cfg), ctypes.byref(h)), "rk_server_create")
+        self._h = h.value
+
+    def submit(self, vals, k: int, out_vals, out_idx, length: int | None = None) -> int:
+        """Queues top-k of the device row ``vals``; returns the ticket."""
+        n = vals.numel() if length is None else int(length)
+        t = ctypes.c_uint64()
+        _check(lib().rk_server_submit(self._h, _ptr(vals), n, int(k), _ptr(out_vals),
+                                      _ptr(out_idx), ctypes.byref(t)), "rk_server_submit")
+        return t.value
+
+    def wait(self, ticket: int) -> float:
+        lat = ctypes.c_double()
+        _check(lib().rk_server_wait(self._h, int(ticket), ctypes.byref(lat)), "rk_server_wait")
+        return lat.value
+
+    def wait_code(self, ticket: int) -> int:
+        """Return code of the request (0, or an RK_E_* code) instead of raising."""
+        return lib().rk_server_wait(self._h, int(ticket), None)
+
+    def drain(self) -> None:
+        _check(lib().rk_server_drain(self._h), "rk_server_drain")
+
+    def replay(self, val_ptrs, lens, ks, out_val_ptrs, out_idx_ptrs, arrivals):
+        """Submits request i at arrivals[i] s (open loop); returns (latency_us, makespan_s)."""
+        n = len(lens)
+        P = np.ascontiguousarray(np.asarray(val_ptrs, dtype=np.uint64))
+        L = np.ascontiguousarray(np.asarray(lens, dtype=np.int64))
+        K = np.ascontiguousarray(np.asarray(ks, dtype=np.int32))
+        OV = np.ascontiguousarray(np.asarray(out_val_ptrs, dtype=np.uint64))
+        OI = np.ascontiguousarray(np.asarray(out_idx_ptrs, dtype=np.uint64))
+        A = np.ascontiguousarray(np.asarray(arrivals, dtype=np.float64))
+        lat = np.zeros(n, np.float64)
+        mk = ctypes.c_double()
+        _check(lib().rk_server_replay(self._h, n, P.ctypes.data, L.ctypes.data, K.ctypes.data,
+                                      OV.ctypes.data, OI.ctypes.data, A.ctypes.data,
+                                      lat.ctypes.data, ctypes.byref(mk)), "rk_server_replay")
+        return lat, mk.value
+
+    def stats(self) -> dict:
+        st = _ServerStats()
+        _check(lib().rk_server_get_stats(self._h, ctypes.byref(st)), "rk_server_get_stats")
+        return {n: getattr(st, n) for n, _ in _ServerStats._fields_ if n != "pad_"}
+
+    def close(self) -> None:
+        if self._h:
+            lib().rk_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -24,6 +24,7 @@ SOURCES = [
     "rk_api.cpp",
     "rk_gen.hip",
     "rk_tune.cpp",
+    "rk_serve.cpp",
 ]
 
 COMMON = [

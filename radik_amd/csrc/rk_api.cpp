@@ -11,7 +11,8 @@
 #include "rk_pipeline.hpp"
 
 namespace rk {
-hipError_t run_pipeline(int dtype, const KParams& p, hipStream_t stream, int row_grid, int grid);
+hipError_t run_pipeline(int dtype, const KParams& p, hipStream_t stream, int row_grid, int grid,
+                        int part);
 int lds_keys_max();
 int timing_enable(int on);
 int timing_read(double* ms, int32_t* launches);
@@ -123,7 +124,8 @@ struct Outputs {
 int launch(int dtype, const void* vals, const int64_t* row_off, const void* const* row_ptrs,
            const int64_t* row_len, const int32_t* row_k, int32_t K, int32_t num_rows,
            int64_t total_len, int32_t max_k, uint32_t flags, const int32_t* idx_in,
-           const Outputs& out, void* workspace, size_t ws_bytes, void* stream) {
+           const Outputs& out, void* workspace, size_t ws_bytes, void* stream,
+           int part = rk::PART_ALL) {
   if (!valid_dtype(dtype)) return RK_E_UNSUPPORTED;
   if (num_rows < 0 || total_len < 0 || max_k < 0) return RK_E_ARG;
   if (num_rows == 0) return RK_OK;
@@ -184,11 +186,34 @@ int launch(int dtype, const void* vals, const int64_t* row_off, const void* cons
   const int row_grid = std::min(num_rows, 8192);
   const uint64_t tiles_bound = (uint64_t)total_len / (uint64_t)p.tune.tile + (uint64_t)num_rows;
   const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)p.tune.grid, tiles_bound));
-  const hipError_t e = rk::run_pipeline(dtype, p, static_cast<hipStream_t>(stream), row_grid, grid);
+  const hipError_t e =
+      rk::run_pipeline(dtype, p, static_cast<hipStream_t>(stream), row_grid, grid, part);
   return e == hipSuccess ? RK_OK : RK_E_HIP;
 }
 
 }  // namespace
+
+namespace rk {
+
+// rk_topk_rows issuing one part of the launch sequence (serving loop).  The
+// two parts of one call must be issued with identical arguments, in order,
+// on one stream; the workspace header (its first 256 bytes) tells between
+// them whether PART_FALLBACK has work.
+int topk_rows_part(int dtype, const void* const* row_ptrs, const int64_t* row_len,
+                   const int32_t* row_k, int32_t num_rows, int64_t total_len, int32_t max_k,
+                   uint32_t flags, void* const* out_val_ptrs, int32_t* const* out_idx_ptrs,
+                   void* workspace, size_t ws_bytes, void* stream, int part) {
+  if (!valid_dtype(dtype)) return RK_E_UNSUPPORTED;
+  if (!row_ptrs || !row_len || !row_k || !out_val_ptrs || !out_idx_ptrs) return RK_E_ARG;
+  if (part != PART_ALL && (flags & RK_EXACT)) return RK_E_ARG;
+  Outputs o;
+  o.val_ptrs = out_val_ptrs;
+  o.idx_ptrs = out_idx_ptrs;
+  return launch(dtype, nullptr, nullptr, row_ptrs, row_len, row_k, 0, num_rows, total_len,
+                max_k, flags, nullptr, o, workspace, ws_bytes, stream, part);
+}
+
+}  // namespace rk
 
 extern "C" {
 

@@ -1101,6 +1101,7 @@ struct Launch {
   hipStream_t stream;
   int row_grid;   // grid of per-row kernels
   int grid;       // grid of streaming kernels
+  int part;       // PART_ALL, or one half of the sequence (PART_MAIN / PART_FALLBACK)
 };
 
 template <int DT, bool LARGEST>
@@ -1190,9 +1191,11 @@ static hipError_t run_typed_u(const KParams& p, const Launch& L0) {
   const int g_ford = rows(resident_grid(k_finalize_ordered<DT, LARGEST>, SEL_NT, dyn));
   KParams p1 = p;  // fallback round: kernels return at once when no row fell back
   p1.round = 1;
-  k_plan_rows<DT><<<rows_blocks, 256, 0, s>>>(p);
-  k_plan_scan<<<1, 1024, 0, s>>>(p);
-  if (!(p.flags & FL_EXACT)) {
+  if (L0.part != PART_FALLBACK) {
+    k_plan_rows<DT><<<rows_blocks, 256, 0, s>>>(p);
+    k_plan_scan<<<1, 1024, 0, s>>>(p);
+  }
+  if (!(p.flags & FL_EXACT) && L0.part != PART_FALLBACK) {
     k_sample<DT, LARGEST><<<g_sample, SEL_NT, dyn_sample, s>>>(p);
     std::pair<hipEvent_t, hipEvent_t> ev;
     const bool timed = timing_begin(s, ev);
@@ -1200,6 +1203,7 @@ static hipError_t run_typed_u(const KParams& p, const Launch& L0) {
     if (timed) timing_end(s, ev);
     k_select_finalize<DT, LARGEST><<<g_select, SEL_NT, dyn, s>>>(p);
   }
+  if (L0.part == PART_MAIN) return hipGetLastError();
   k_exact_init<<<g_init, 256, 0, s>>>(p1);
   const int passes = (KB + p.tune.hist_bits - 1) / p.tune.hist_bits;
   for (int i = 0; i < passes; ++i) {
@@ -1229,8 +1233,10 @@ static hipError_t run_typed(const KParams& p, const Launch& L) {
 }
 
 // Entry used by the C-ABI layer.
-hipError_t run_pipeline(int dtype, const KParams& p, hipStream_t stream, int row_grid, int grid) {
-  Launch L{stream, row_grid, grid};
+hipError_t run_pipeline(int dtype, const KParams& p, hipStream_t stream, int row_grid, int grid,
+                        int part) {
+  if (part != PART_ALL && (p.flags & FL_EXACT)) return hipErrorInvalidValue;
+  Launch L{stream, row_grid, grid, part};
   const bool largest = p.flags & FL_LARGEST;
   switch (dtype) {
     case F32: return largest ? run_typed<F32, true>(p, L) : run_typed<F32, false>(p, L);

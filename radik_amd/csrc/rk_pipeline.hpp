@@ -134,4 +134,11 @@ enum : uint32_t {
 constexpr int HIST_MAX_BINS = 2048;
 constexpr int TILE_MIN = 1024;
 
+// Which part of the launch sequence a call issues.  PART_MAIN is the sampled
+// round (plan, sample, filter, select_finalize); PART_FALLBACK the exact
+// passes and ordered emit that only do work when the header's fallback_rows
+// is non-zero.  A caller that reads the header between the two (the serving
+// loop) skips the idle second half; PART_ALL issues both back to back.
+enum Part : int { PART_ALL = 0, PART_MAIN = 1, PART_FALLBACK = 2 };
+
 }  // namespace rk

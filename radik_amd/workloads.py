@@ -154,6 +154,45 @@ MIX_PARAMS = {0: (UNIFORM, 0.0, 1.0, 0.0), 1: (NORMAL, 0.0, 1.0, 0.0),
               2: (ZIPF, 1.1, 2.0 ** 20, 0.0), 3: (NEARCONST, 0.0, 0.0, 0.3)}
 
 
+def materialize_requests(lens, kinds, seed: int = 5, dtype: str = "f32", device="cuda"):
+    """One device arena holding every request's row (request i at
+    [off[i], off[i+1]), family MIX_PARAMS[kinds[i]], generator stream i);
+    returns (values, off)."""
+    import torch
+
+    import radik_amd as rk
+
+    lens = np.asarray(lens, np.int64)
+    off = np.zeros(len(lens) + 1, np.int64)
+    off[1:] = np.cumsum(lens)
+    tdt = {"f32": torch.float32, "bf16": torch.bfloat16, "f16": torch.float16}[dtype]
+    vals = torch.empty(max(int(off[-1]), 1), dtype=tdt, device=device)
+    for i, kind in enumerate(np.asarray(kinds)):
+        lo, hi = int(off[i]), int(off[i + 1])
+        if hi > lo:
+            g, a, b, dup = MIX_PARAMS[int(kind)]
+            rk.gen_fill(vals[lo:hi], g, seed, i, a=a, b=b, dup_frac=dup, row_len=hi - lo)
+    return vals, off
+
+
+def c5_bodies(per_kind: int = 256, body_len: int = 1 << 22, seed: int = 5, device="cuda"):
+    """Resident request bodies of the serving workload: per_kind fp32 rows of
+    body_len for each of the four families; body (kind, j) is row
+    kind * per_kind + j.  A request of length n reads the first n elements of
+    a body of its family."""
+    import torch
+
+    import radik_amd as rk
+
+    bodies = torch.empty((4 * per_kind, body_len), dtype=torch.float32, device=device)
+    for kind in range(4):
+        g, a, b, dup = MIX_PARAMS[kind]
+        for j in range(per_kind):
+            r = kind * per_kind + j
+            rk.gen_fill(bodies[r], g, seed, 1000 + r, a=a, b=b, dup_frac=dup, row_len=body_len)
+    return bodies
+
+
 def materialize(w: Workload, device="cuda"):
     """Device tensor of the workload's values (generated on the device)."""
     import torch

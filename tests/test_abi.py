@@ -120,3 +120,33 @@ def test_product_never_imports_the_oracle():
         if path.endswith((".py", ".cpp", ".hip", ".hpp", ".h")):
             text = open(path).read()
             assert "oracle" not in text.replace("the oracle", ""), path
+
+
+def test_server_argument_validation():
+    """The serving entry points reject bad configurations and null handles
+    before touching the device."""
+    import ctypes
+
+    import radik_amd as rk
+    L = rk.lib()
+    h = ctypes.c_void_p()
+    cfg = rk._ServerConfig(max_batch_keys=1 << 20, mem_budget=1 << 30, dtype=rk.RK_F32,
+                           flags=rk.RK_LARGEST, max_k=64, max_batch_rows=0, streams=1, plain=0)
+    assert L.rk_server_create(ctypes.byref(cfg), ctypes.byref(h)) == rk.RK_E_ARG
+    cfg.max_batch_rows = 8
+    cfg.flags = rk.RK_LARGEST | rk.RK_EXACT  # the exact path is chosen by `plain`
+    assert L.rk_server_create(ctypes.byref(cfg), ctypes.byref(h)) == rk.RK_E_ARG
+    cfg.flags = rk.RK_LARGEST
+    cfg.dtype = 7
+    assert L.rk_server_create(ctypes.byref(cfg), ctypes.byref(h)) == rk.RK_E_UNSUPPORTED
+    cfg.dtype = rk.RK_F32
+    cfg.mem_budget = 0
+    assert L.rk_server_create(ctypes.byref(cfg), ctypes.byref(h)) == rk.RK_E_ARG
+    assert L.rk_server_create(None, ctypes.byref(h)) == rk.RK_E_ARG
+    t = ctypes.c_uint64()
+    assert L.rk_server_submit(None, None, 0, 0, None, None, ctypes.byref(t)) == rk.RK_E_ARG
+    assert L.rk_server_wait(None, 0, None) == rk.RK_E_ARG
+    assert L.rk_server_drain(None) == rk.RK_E_ARG
+    assert L.rk_server_get_stats(None, None) == rk.RK_E_ARG
+    assert L.rk_server_destroy(None) == rk.RK_E_ARG
+    assert ctypes.sizeof(rk._ServerConfig) == 40 and ctypes.sizeof(rk._ServerStats) == 104
