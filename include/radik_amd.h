@@ -117,8 +117,12 @@ int rk_topk_status(const void* workspace, void* stream, uint32_t* status,
 /* Human-readable text of a return code. */
 const char* rk_strerror(int code);
 
-/* Tunables of the batched path (tile size, persistent grid, sample shape,
- * digit width, loads in flight).  Values <= 0 keep the default. */
+/* Tunables of the batched path: tile size (keys per streaming work unit,
+ * default 16384), persistent grid of the streaming kernels (default 8
+ * workgroups per CU), sample runs and keys per run (1024 x 16), digit width
+ * of the exact passes (8..11 bits, default 11: 3 passes over 32-bit keys)
+ * and 16-byte loads in flight per thread (2, 4 or 8; default 4).  A field of
+ * 0 selects its default. */
 typedef struct rk_tune {
   int32_t tile;
   int32_t grid;
@@ -129,7 +133,8 @@ typedef struct rk_tune {
 } rk_tune;
 
 /* Reads / overrides the tunables used by subsequent calls in this process
- * (the autotuner sets them per shape bucket). */
+ * whose shape bucket has no autotuned entry.  rk_set_tune leaves fields < 0
+ * unchanged; rk_get_tune reports 0 for a field at its default. */
 int rk_get_tune(rk_tune* out);
 int rk_set_tune(const rk_tune* in);
 

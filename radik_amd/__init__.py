@@ -28,7 +28,8 @@ __all__ = [
     "RadikError", "lib", "lib_path", "max_k", "get_workspace_size", "workspace_size",
     "topk_radix_select", "topk_batched", "topk_rows", "status", "gen_fill",
     "get_tune", "set_tune", "timing_enable", "timing_read", "autotune_batched",
-    "tune_cache_save", "tune_cache_load", "tune_cache_clear", "Graph", "Server",
+    "tune_cache_save", "tune_cache_load", "tune_cache_clear", "tune_cache_size", "Graph",
+    "Server",
 ]
 
 RK_F32, RK_F16, RK_BF16 = 0, 1, 2
@@ -353,6 +354,11 @@ def tune_cache_clear() -> None:
     _check(lib().rk_tune_cache_clear(), "rk_tune_cache_clear")
 
 
+def tune_cache_size() -> int:
+    """Number of shape buckets with an autotuned entry."""
+    return int(lib().rk_tune_cache_size())
+
+
 class Graph:
     """A hipGraph of radik_amd calls captured on a (non-null) stream:
 
@@ -417,7 +423,8 @@ def get_tune() -> dict:
 
 
 def set_tune(**kw) -> None:
-    t = _Tune(**{n: int(kw.get(n, 0)) for n, _ in _Tune._fields_})
+    """Sets the named tunables (0 = that field's default); others keep their value."""
+    t = _Tune(**{n: int(kw.get(n, -1)) for n, _ in _Tune._fields_})
     _check(lib().rk_set_tune(ctypes.byref(t)), "rk_set_tune")
 
 

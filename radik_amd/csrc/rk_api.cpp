@@ -25,7 +25,7 @@ using rk::KParams;
 using rk::Tune;
 
 std::mutex g_mu;
-Tune g_tune{16384, 0, 1024, 16, 11, 4};
+Tune g_tune{0, 0, 0, 0, 0, 0};  // 0: the default of each field (current_tune)
 
 struct DevInfo {
   int cus = 0;
@@ -101,12 +101,13 @@ Tune current_tune(const DevInfo& di, int dtype, uint32_t flags, int32_t rows, in
     std::lock_guard<std::mutex> lk(g_mu);
     t = g_tune;
   }
+  if (t.tile <= 0) t.tile = 16384;
   if (t.tile < rk::TILE_MIN) t.tile = rk::TILE_MIN;
   t.tile = (t.tile / 64) * 64;
   if (t.grid <= 0) t.grid = di.cus * 8;
   if (t.sample_groups <= 0) t.sample_groups = 1024;
   if (t.group_elems <= 0) t.group_elems = 16;
-  t.group_elems = std::max(8, (t.group_elems / 8) * 8);
+  t.group_elems = std::min(256, std::max(8, (t.group_elems / 8) * 8));
   if (t.hist_bits < 8 || t.hist_bits > 11) t.hist_bits = 11;
   if (t.unroll != 2 && t.unroll != 4 && t.unroll != 8) t.unroll = 4;
   return t;
@@ -181,7 +182,9 @@ int launch(int dtype, const void* vals, const int64_t* row_off, const void* cons
   p.lds_keys = std::min(lk, di.lds_keys);
   p.small_max = p.lds_keys;
   p.tune = current_tune(di, dtype, flags, num_rows, total_len, max_k);
-  p.sample_keys = std::min(16384, p.tune.sample_groups * p.tune.group_elems);
+  // at least 64 runs fit (the sample's minimum), at most 16384 keys of LDS
+  p.sample_keys = (int32_t)std::min<int64_t>(
+      16384, (int64_t)std::max(p.tune.sample_groups, 64) * p.tune.group_elems);
   p.round = 0;
   const int row_grid = std::min(num_rows, 8192);
   const uint64_t tiles_bound = (uint64_t)total_len / (uint64_t)p.tune.tile + (uint64_t)num_rows;
@@ -350,12 +353,13 @@ int rk_get_tune(rk_tune* out) {
 int rk_set_tune(const rk_tune* in) {
   if (!in) return RK_E_ARG;
   std::lock_guard<std::mutex> lk(g_mu);
-  if (in->tile > 0) g_tune.tile = in->tile;
-  if (in->grid > 0) g_tune.grid = in->grid;
-  if (in->sample_groups > 0) g_tune.sample_groups = in->sample_groups;
-  if (in->group_elems > 0) g_tune.group_elems = in->group_elems;
-  if (in->hist_bits > 0) g_tune.hist_bits = in->hist_bits;
-  if (in->unroll > 0) g_tune.unroll = in->unroll;
+  // < 0 keeps the current value; 0 selects the default
+  if (in->tile >= 0) g_tune.tile = in->tile;
+  if (in->grid >= 0) g_tune.grid = in->grid;
+  if (in->sample_groups >= 0) g_tune.sample_groups = in->sample_groups;
+  if (in->group_elems >= 0) g_tune.group_elems = in->group_elems;
+  if (in->hist_bits >= 0) g_tune.hist_bits = in->hist_bits;
+  if (in->unroll >= 0) g_tune.unroll = in->unroll;
   return RK_OK;
 }
 

@@ -305,11 +305,11 @@ __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
     const int64_t len = d.len;
     const int64_t ge = p.tune.group_elems;
     int64_t ng = p.tune.sample_groups;
-    const int64_t max_keys = p.sample_keys;
-    if (ng * ge > max_keys) ng = max_keys / ge;
+    const int64_t ng_cap = p.sample_keys / ge;      // runs that fit the LDS keys
     if (ng * ge * 64 > len) ng = len / (64 * ge);  // read about 1/64 of the row,
     const int64_t ng_min = len / (2 * ge) < 64 ? len / (2 * ge) : 64;
-    if (ng < ng_min) ng = ng_min;                   // but at least 64 runs
+    if (ng < ng_min) ng = ng_min;                   // but at least 64 runs,
+    if (ng > ng_cap) ng = ng_cap;                   // and never beyond the LDS
     if (ng < 1) ng = 1;
     const uint32_t s = (uint32_t)(ng * ge);
     constexpr int UN = 8;  // loads in flight per thread

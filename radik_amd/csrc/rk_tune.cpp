@@ -83,12 +83,22 @@ int rk_autotune_batched(int dtype, const void* vals, const int64_t* row_off,
   }
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return RK_E_HIP;
+  // Candidates: tile size x persistent grid (blocks per CU) x loads in
+  // flight, and per path the parameter that shapes its passes: the sample
+  // size for the sampled path, the digit width (hence the pass count, e.g.
+  // 3 x 11 or 4 x 8 bits for 32-bit keys) for the exact path.
   std::vector<rk_tune> cands;
+  const bool exact = (flags & RK_EXACT) != 0;
   for (int tile : {4096, 16384, 65536})
     for (int gm : {4, 8, 16})
-      for (int un : {2, 4})
-        for (int sg : {512, 2048})
-          cands.push_back(rk_tune{tile, gm * cus, sg, 16, saved.hist_bits, un});
+      for (int un : {2, 4}) {
+        if (exact) {
+          for (int hb : {8, 11}) cands.push_back(rk_tune{tile, gm * cus, 1024, 16, hb, un});
+        } else {
+          for (int sg : {512, 2048})
+            cands.push_back(rk_tune{tile, gm * cus, sg, 16, saved.hist_bits, un});
+        }
+      }
   float best_t = 1e30f;
   rk_tune best_c = saved;
   int rc = RK_OK;
