@@ -204,6 +204,7 @@ typedef struct rk_server_stats {
   int64_t server_bytes, peak_inflight_bytes;
   double mean_batch_rows;
   double p50_us, p99_us, mean_us, max_us; /* latency: arrival to observed completion */
+  double issue_us;                        /* mean host time to issue one batch part */
   int32_t max_batch_rows;
   int32_t pad_;
 } rk_server_stats;

@@ -12,7 +12,7 @@
 
 namespace rk {
 hipError_t run_pipeline(int dtype, const KParams& p, hipStream_t stream, int row_grid, int grid,
-                        int part);
+                        int part, uint32_t* publish);
 int lds_keys_max();
 int timing_enable(int on);
 int timing_read(double* ms, int32_t* launches);
@@ -101,7 +101,14 @@ Tune current_tune(const DevInfo& di, int dtype, uint32_t flags, int32_t rows, in
     std::lock_guard<std::mutex> lk(g_mu);
     t = g_tune;
   }
-  if (t.tile <= 0) t.tile = 16384;
+  if (t.tile <= 0) {
+    // Default 16384 keys; a call too small to give every CU two tiles gets
+    // smaller tiles (down to the minimum) so that its streaming kernels
+    // still spread over the chip (a single serving request, a short batch).
+    int64_t tl = 16384;
+    while (tl > rk::TILE_MIN && total / tl < 2 * (int64_t)di.cus) tl >>= 1;
+    t.tile = (int32_t)tl;
+  }
   if (t.tile < rk::TILE_MIN) t.tile = rk::TILE_MIN;
   t.tile = (t.tile / 64) * 64;
   if (t.grid <= 0) t.grid = di.cus * 8;
@@ -126,7 +133,7 @@ int launch(int dtype, const void* vals, const int64_t* row_off, const void* cons
            const int64_t* row_len, const int32_t* row_k, int32_t K, int32_t num_rows,
            int64_t total_len, int32_t max_k, uint32_t flags, const int32_t* idx_in,
            const Outputs& out, void* workspace, size_t ws_bytes, void* stream,
-           int part = rk::PART_ALL) {
+           int part = rk::PART_ALL, uint32_t* publish = nullptr) {
   if (!valid_dtype(dtype)) return RK_E_UNSUPPORTED;
   if (num_rows < 0 || total_len < 0 || max_k < 0) return RK_E_ARG;
   if (num_rows == 0) return RK_OK;
@@ -190,7 +197,7 @@ int launch(int dtype, const void* vals, const int64_t* row_off, const void* cons
   const uint64_t tiles_bound = (uint64_t)total_len / (uint64_t)p.tune.tile + (uint64_t)num_rows;
   const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)p.tune.grid, tiles_bound));
   const hipError_t e =
-      rk::run_pipeline(dtype, p, static_cast<hipStream_t>(stream), row_grid, grid, part);
+      rk::run_pipeline(dtype, p, static_cast<hipStream_t>(stream), row_grid, grid, part, publish);
   return e == hipSuccess ? RK_OK : RK_E_HIP;
 }
 
@@ -201,11 +208,14 @@ namespace rk {
 // rk_topk_rows issuing one part of the launch sequence (serving loop).  The
 // two parts of one call must be issued with identical arguments, in order,
 // on one stream; the workspace header (its first 256 bytes) tells between
-// them whether PART_FALLBACK has work.
+// them whether PART_FALLBACK has work.  With PART_MAIN, `publish` (device
+// address of mapped host memory, or null) receives the header's first 8
+// words when the part completes.
 int topk_rows_part(int dtype, const void* const* row_ptrs, const int64_t* row_len,
                    const int32_t* row_k, int32_t num_rows, int64_t total_len, int32_t max_k,
                    uint32_t flags, void* const* out_val_ptrs, int32_t* const* out_idx_ptrs,
-                   void* workspace, size_t ws_bytes, void* stream, int part) {
+                   void* workspace, size_t ws_bytes, void* stream, int part,
+                   uint32_t* publish) {
   if (!valid_dtype(dtype)) return RK_E_UNSUPPORTED;
   if (!row_ptrs || !row_len || !row_k || !out_val_ptrs || !out_idx_ptrs) return RK_E_ARG;
   if (part != PART_ALL && (flags & RK_EXACT)) return RK_E_ARG;
@@ -213,7 +223,8 @@ int topk_rows_part(int dtype, const void* const* row_ptrs, const int64_t* row_le
   o.val_ptrs = out_val_ptrs;
   o.idx_ptrs = out_idx_ptrs;
   return launch(dtype, nullptr, nullptr, row_ptrs, row_len, row_k, 0, num_rows, total_len,
-                max_k, flags, nullptr, o, workspace, ws_bytes, stream, part);
+                max_k, flags, nullptr, o, workspace, ws_bytes, stream, part,
+                part == PART_MAIN ? publish : nullptr);
 }
 
 }  // namespace rk

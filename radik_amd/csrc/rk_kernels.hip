@@ -102,10 +102,8 @@ RK_DEV int64_t aligned_start(const typename Traits<DT>::U* rb, int64_t es) {
 // plan
 // ---------------------------------------------------------------------------
 template <int DT>
-__global__ __launch_bounds__(256) void k_plan_rows(KParams p) {
+RK_DEV void plan_row(const KParams& p, int r) {
   using U = typename Traits<DT>::U;
-  const int r = blockIdx.x * 256 + threadIdx.x;
-  if (r >= p.R) return;
   const U* base;
   const int32_t* idx = nullptr;
   int64_t len;
@@ -172,9 +170,22 @@ __global__ __launch_bounds__(256) void k_plan_rows(KParams p) {
   p.cand_cnt[r] = 0;
 }
 
+template <int DT>
+__global__ __launch_bounds__(256) void k_plan_rows(KParams p) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r < p.R) plan_row<DT>(p, r);
+}
+
 // Exclusive scans of the per-row tile counts and candidate capacities (one
 // workgroup; rows are processed in chunks of 1024 with a running carry).
+// FUSED (at most 1024 rows): the same workgroup plans the rows first, which
+// saves a launch on small batches.
+template <int DT, bool FUSED>
 __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
+  if (FUSED) {
+    if ((int)threadIdx.x < p.R) plan_row<DT>(p, threadIdx.x);
+    __syncthreads();
+  }
   __shared__ uint32_t scratch[16];
   __shared__ uint64_t scratch64[16];
   __shared__ uint32_t bad_sh;
@@ -232,6 +243,7 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
     h->q_sample = 0;
     h->q_select[0] = 0;
     h->q_select[1] = 0;
+    h->sel_done = 0;
     if (carry_t > p.tile_capacity) status |= ST_TILES;
     if (carry_c > p.cand_capacity) status |= ST_CANDIDATES;
     h->status = status;
@@ -737,6 +749,18 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
     if (threadIdx.x == 0) p.st[r].phase = PH_DONE;
     __syncthreads();
   }
+  if (p.publish && threadIdx.x == 0) {
+    // the last workgroup to finish copies the header (every other
+    // workgroup's header updates precede its increment of sel_done)
+    __threadfence();
+    if (atomicAdd(&p.hdr->sel_done, 1u) == gridDim.x - 1) {
+      __threadfence();
+      const uint32_t* h = reinterpret_cast<const uint32_t*>(p.hdr);
+      for (int i = 0; i < 8; ++i)
+        p.publish[i] = __hip_atomic_load(h + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __threadfence_system();
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1102,6 +1126,7 @@ struct Launch {
   int row_grid;   // grid of per-row kernels
   int grid;       // grid of streaming kernels
   int part;       // PART_ALL, or one half of the sequence (PART_MAIN / PART_FALLBACK)
+  uint32_t* publish;  // PART_MAIN: header words to mapped host memory (or null)
 };
 
 template <int DT, bool LARGEST>
@@ -1192,8 +1217,12 @@ static hipError_t run_typed_u(const KParams& p, const Launch& L0) {
   KParams p1 = p;  // fallback round: kernels return at once when no row fell back
   p1.round = 1;
   if (L0.part != PART_FALLBACK) {
-    k_plan_rows<DT><<<rows_blocks, 256, 0, s>>>(p);
-    k_plan_scan<<<1, 1024, 0, s>>>(p);
+    if (p.R <= 1024) {
+      k_plan_scan<DT, true><<<1, 1024, 0, s>>>(p);
+    } else {
+      k_plan_rows<DT><<<rows_blocks, 256, 0, s>>>(p);
+      k_plan_scan<DT, false><<<1, 1024, 0, s>>>(p);
+    }
   }
   if (!(p.flags & FL_EXACT) && L0.part != PART_FALLBACK) {
     k_sample<DT, LARGEST><<<g_sample, SEL_NT, dyn_sample, s>>>(p);
@@ -1201,7 +1230,9 @@ static hipError_t run_typed_u(const KParams& p, const Launch& L0) {
     const bool timed = timing_begin(s, ev);
     k_filter<DT, LARGEST, U><<<fgrid, STREAM_NT, 0, s>>>(p);
     if (timed) timing_end(s, ev);
-    k_select_finalize<DT, LARGEST><<<g_select, SEL_NT, dyn, s>>>(p);
+    KParams pm = p;
+    pm.publish = L0.part == PART_MAIN ? L0.publish : nullptr;
+    k_select_finalize<DT, LARGEST><<<g_select, SEL_NT, dyn, s>>>(pm);
   }
   if (L0.part == PART_MAIN) return hipGetLastError();
   k_exact_init<<<g_init, 256, 0, s>>>(p1);
@@ -1234,9 +1265,9 @@ static hipError_t run_typed(const KParams& p, const Launch& L) {
 
 // Entry used by the C-ABI layer.
 hipError_t run_pipeline(int dtype, const KParams& p, hipStream_t stream, int row_grid, int grid,
-                        int part) {
+                        int part, uint32_t* publish) {
   if (part != PART_ALL && (p.flags & FL_EXACT)) return hipErrorInvalidValue;
-  Launch L{stream, row_grid, grid, part};
+  Launch L{stream, row_grid, grid, part, publish};
   const bool largest = p.flags & FL_LARGEST;
   switch (dtype) {
     case F32: return largest ? run_typed<F32, true>(p, L) : run_typed<F32, false>(p, L);
@@ -1247,6 +1278,20 @@ hipError_t run_pipeline(int dtype, const KParams& p, hipStream_t stream, int row
 }
 
 int lds_keys_max() { return LDS_KEYS_MAX; }
+
+// Copies the first 32 bytes of a workspace header (status, totals, fallback
+// and ordered row counts) to `dst`, which may be mapped host memory: a
+// stream-ordered way for a host loop to learn a call's outcome from the
+// completion of this kernel alone, without a copy engine.
+__global__ __launch_bounds__(64) void k_publish(const uint32_t* hdr, uint32_t* dst) {
+  if (threadIdx.x < 8) dst[threadIdx.x] = hdr[threadIdx.x];
+  __threadfence_system();
+}
+
+hipError_t publish_header(const void* workspace, uint32_t* dst, hipStream_t stream) {
+  k_publish<<<1, 64, 0, stream>>>(static_cast<const uint32_t*>(workspace), dst);
+  return hipGetLastError();
+}
 
 int timing_enable(int on) {
   TimingState& t = timing();

@@ -70,7 +70,8 @@ struct Header {
   uint32_t ordered_rows;   // rows that needed the ordered emit
   uint32_t q_sample;       // row queue heads of the per-row kernels
   uint32_t q_select[2];    //   (select: sampled round, fallback round)
-  uint32_t pad[55];
+  uint32_t sel_done;       // workgroups of the sampled-round select that finished
+  uint32_t pad[54];
 };
 static_assert(sizeof(Header) == 256, "workspace header is 256 bytes");
 
@@ -121,6 +122,9 @@ struct KParams {
   int32_t sample_keys;  // LDS key capacity of the sample kernel (this launch)
   int32_t round;        // 0: sampled round; 1: fallback round (skips when idle)
   Tune tune;
+  // When set (sampled-round select only): the last select workgroup to finish
+  // copies the header's first 8 words here (mapped host memory).
+  uint32_t* publish;
 };
 
 // Public flag bits (mirrors of the C-ABI values).

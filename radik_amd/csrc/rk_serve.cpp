@@ -11,10 +11,14 @@
 //    (its own stream, workspace and metadata).  Up to `streams` batches run at
 //    once, so requests that arrive while others are on the GPU start as soon
 //    as a slot and budget free up, and their work overlaps on the device.
-//  * A batch issues the sampled round first, then copies the workspace header
-//    to pinned host memory; only when it reports rows that fell back to the
-//    exact passes is the second half of the sequence issued.  Completion is
-//    detected from a per-slot HIP event polled by the dispatcher.
+//  * The batch's metadata is written by the host into mapped pinned memory
+//    that the planning kernel reads directly (a host-to-device copy would
+//    wait tens of microseconds on the copy engine).  A batch issues the
+//    sampled round first, whose last select workgroup copies the workspace
+//    header into mapped host memory; only when it reports rows that fell
+//    back to the exact passes is the second half of the sequence issued.
+//    Completion is detected from a per-slot HIP event polled by the
+//    dispatcher.
 //  * Latency of a request = host time at which its completion was observed
 //    minus its arrival time.
 //  * plain = 1 is the baseline the loop is measured against: one request per
@@ -27,6 +31,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstddef>
 #include <cstring>
 #include <deque>
 #include <mutex>
@@ -40,7 +45,9 @@ namespace rk {
 int topk_rows_part(int dtype, const void* const* row_ptrs, const int64_t* row_len,
                    const int32_t* row_k, int32_t num_rows, int64_t total_len, int32_t max_k,
                    uint32_t flags, void* const* out_val_ptrs, int32_t* const* out_idx_ptrs,
-                   void* workspace, size_t ws_bytes, void* stream, int part);
+                   void* workspace, size_t ws_bytes, void* stream, int part,
+                   uint32_t* publish);
+hipError_t publish_header(const void* workspace, uint32_t* dst, hipStream_t stream);
 }  // namespace rk
 
 namespace {
@@ -96,6 +103,9 @@ Meta carve_meta(void* base, int32_t rows) {
   return m;
 }
 
+static_assert(offsetof(rk::Header, status) == 0 && offsetof(rk::Header, fallback_rows) == 16,
+              "published header words: 0 status, 4 fallback rows");
+
 enum SlotStage : int { SL_IDLE = 0, SL_MAIN = 1, SL_FALLBACK = 2, SL_ALL = 3 };
 
 struct Slot {
@@ -103,9 +113,10 @@ struct Slot {
   hipEvent_t done = nullptr;
   void* ws = nullptr;
   size_t ws_bytes = 0;
-  void* d_meta = nullptr;
-  void* h_meta = nullptr;       // pinned staging of the metadata
-  rk::Header* h_hdr = nullptr;  // pinned copy of the workspace header
+  void* h_meta = nullptr;   // mapped pinned memory: the batch's metadata
+  void* d_meta = nullptr;   //   and its device address
+  uint32_t* h_hdr = nullptr;  // mapped pinned memory: first 32 bytes of the header
+  uint32_t* d_hdr = nullptr;  //   and its device address
   int stage = SL_IDLE;
   // the batch in flight (owned by the dispatcher thread)
   std::vector<uint64_t> tickets;
@@ -135,6 +146,8 @@ struct rk_server {
   bool stop = false;
   int error = RK_OK;
   int64_t batches = 0, batched_rows = 0, fallback_batches = 0, rejected = 0, failed = 0;
+  double issue_s = 0.0;  // host time spent issuing batch parts
+  int64_t issues = 0;
   int32_t max_rows_seen = 0;
 
   std::thread dispatcher;
@@ -149,18 +162,24 @@ struct rk_server {
   }
 };
 
-// Issues one part of the slot's batch, then the header copy and the event.
+// Issues one part of the slot's batch, then the event.  The header reaches
+// the slot's mapped host words from the sampled round's last kernel, or from
+// a one-wave kernel after the other parts.
 int rk_server::issue(Slot& s, int part) {
+  const double t0 = now_s();
   Meta d = carve_meta(s.d_meta, cfg.max_batch_rows);
   int rc = rk::topk_rows_part(cfg.dtype, d.ptrs, d.lens, d.ks, s.n, s.keys, std::max(s.max_k, 1),
                               flags, reinterpret_cast<void* const*>(d.ovals),
                               reinterpret_cast<int32_t* const*>(d.oidx), s.ws, s.ws_bytes,
-                              s.stream, part);
+                              s.stream, part, s.d_hdr);
   if (rc != RK_OK) return rc;
-  if (hipMemcpyAsync(s.h_hdr, s.ws, sizeof(rk::Header), hipMemcpyDeviceToHost, s.stream) !=
-          hipSuccess ||
-      hipEventRecord(s.done, s.stream) != hipSuccess)
+  if (part != rk::PART_MAIN && rk::publish_header(s.ws, s.d_hdr, s.stream) != hipSuccess)
     return RK_E_HIP;
+  if (hipEventRecord(s.done, s.stream) != hipSuccess) return RK_E_HIP;
+  const double dt = now_s() - t0;
+  std::lock_guard<std::mutex> lk(mu);
+  issue_s += dt;
+  ++issues;
   return RK_OK;
 }
 
@@ -220,9 +239,7 @@ bool rk_server::start_batch(Slot& s) {
   s.tickets = std::move(take);
   const int part = cfg.plain ? rk::PART_ALL : rk::PART_MAIN;
   s.stage = cfg.plain ? SL_ALL : SL_MAIN;
-  int rc = hipMemcpyAsync(s.d_meta, s.h_meta, meta_bytes(cfg.max_batch_rows),
-                          hipMemcpyHostToDevice, s.stream) == hipSuccess ? RK_OK : RK_E_HIP;
-  if (rc == RK_OK) rc = issue(s, part);
+  const int rc = issue(s, part);
   if (rc != RK_OK) {
     set_error(rc);
     (void)hipStreamSynchronize(s.stream);
@@ -266,10 +283,12 @@ void rk_server::loop() {
         finish(s, false);
         continue;
       }
-      const rk::Header h = *s.h_hdr;
-      if (h.status != 0) {
+      // words of the published header: 0 status, 4 fallback rows
+      const uint32_t status = reinterpret_cast<volatile uint32_t*>(s.h_hdr)[0];
+      const uint32_t fallback_rows = reinterpret_cast<volatile uint32_t*>(s.h_hdr)[4];
+      if (status != 0) {
         finish(s, false);  // a row failed device-side validation
-      } else if (s.stage == SL_MAIN && h.fallback_rows > 0) {
+      } else if (s.stage == SL_MAIN && fallback_rows > 0) {
         // rows that left the sampled path: issue the exact passes
         {
           std::lock_guard<std::mutex> lk(mu);
@@ -332,7 +351,6 @@ void destroy_slots(rk_server* s) {
   for (Slot& sl : s->slots) {
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
     if (sl.ws) (void)hipFree(sl.ws);
-    if (sl.d_meta) (void)hipFree(sl.d_meta);
     if (sl.h_meta) (void)hipHostFree(sl.h_meta);
     if (sl.h_hdr) (void)hipHostFree(sl.h_hdr);
     if (sl.done) (void)hipEventDestroy(sl.done);
@@ -381,10 +399,14 @@ int rk_server_create(const rk_server_config* cfg, rk_server** out) {
     // the caller's work
     if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess ||
-        hipMalloc(&sl.ws, ws) != hipSuccess || hipMalloc(&sl.d_meta, mb) != hipSuccess ||
-        hipHostMalloc(&sl.h_meta, mb, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void**>(&sl.h_hdr), sizeof(rk::Header),
-                      hipHostMallocDefault) != hipSuccess) {
+        hipMalloc(&sl.ws, ws) != hipSuccess ||
+        hipHostMalloc(&sl.h_meta, mb, hipHostMallocMapped | hipHostMallocCoherent) !=
+            hipSuccess ||
+        hipHostGetDevicePointer(&sl.d_meta, sl.h_meta, 0) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&sl.h_hdr), 32,
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void**>(&sl.d_hdr), sl.h_hdr, 0) !=
+            hipSuccess) {
       destroy_slots(s);
       delete s;
       return RK_E_HIP;
@@ -512,6 +534,7 @@ int rk_server_get_stats(rk_server* s, rk_server_stats* out) {
   out->mean_us = lat.empty() ? 0.0 : sum / (double)lat.size();
   out->max_us = lat.empty() ? 0.0 : lat.back();
   out->max_batch_rows = s->max_rows_seen;
+  out->issue_us = s->issues ? s->issue_s / (double)s->issues * 1e6 : 0.0;
   return RK_OK;
 }
 
