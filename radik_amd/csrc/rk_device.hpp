@@ -148,6 +148,11 @@ RK_DEV uint4 bload16(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
   const rk_v4u v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)byte_off, 0, 0);
   return make_uint4(v.x, v.y, v.z, v.w);
 }
+// Non-temporal (streaming) variant for data read once.
+RK_DEV uint4 bload16_nt(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+  const rk_v4u v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)byte_off, 0, 2);
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
 
 // Best value of a 16-byte vector in the selection direction (max for
 // largest-k, min for smallest-k), as fp32.  max/min ignore a NaN operand, so

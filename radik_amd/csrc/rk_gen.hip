@@ -100,26 +100,31 @@ __global__ void k_gen(int dtype, int kind, uint64_t base, int64_t n, int64_t row
 // known-size reference that scales the profiler's fetch counter.
 __global__ __launch_bounds__(256) void k_probe_read(const uint4* p, int64_t nvec,
                                                     unsigned int* out) {
-  const int64_t G = gridDim.x;
-  const int64_t v0 = nvec * blockIdx.x / G, v1 = nvec * (blockIdx.x + 1) / G;
-  const uint32_t n = (uint32_t)(v1 - v0);  // < 2^28 vectors per slice
-  const uint64_t a = (uint64_t)(p + v0);
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
-  const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
-      (void*)(((uint64_t)hi << 32) | lo), (short)0,
-      (int)__builtin_amdgcn_readfirstlane(n * 16u), 0x00020000);
+  // Workgroups take 256 KB blocks grid-stride, so the blocks read at any
+  // moment lie close together; non-temporal loads (nothing is re-read).
+  // The fastest read pattern measured on MI355X.
+  constexpr int U = 4;
+  constexpr int64_t BLK = 16384;  // vectors per block
+  typedef unsigned int v4u __attribute__((ext_vector_type(4)));
   const uint32_t wb = __builtin_amdgcn_readfirstlane(threadIdx.x & ~63u);
   const uint32_t lane = threadIdx.x & 63u;
   uint32_t acc = 0;
-  for (uint32_t jb = wb; jb < n; jb += 256 * 8) {
-    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
-    v4u x[8];
+  for (int64_t b0 = (int64_t)blockIdx.x * BLK; b0 < nvec; b0 += (int64_t)gridDim.x * BLK) {
+    const uint32_t n = (uint32_t)((nvec - b0) < BLK ? (nvec - b0) : BLK);
+    const uint64_t a = (uint64_t)(p + b0);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(((uint64_t)hi << 32) | lo), (short)0,
+        (int)__builtin_amdgcn_readfirstlane(n * 16u), 0x00020000);
+    for (uint32_t jb = wb; jb < n; jb += 256 * U) {
+      v4u x[U];
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
-      x[u] = __builtin_amdgcn_raw_buffer_load_b128(r, (int)((jb + lane + u * 256) * 16u), 0, 0);
+      for (int u = 0; u < U; ++u)
+        x[u] = __builtin_amdgcn_raw_buffer_load_b128(r, (int)((jb + lane + u * 256) * 16u), 0, 2);
 #pragma unroll
-    for (int u = 0; u < 8; ++u) acc ^= x[u].x ^ x[u].y ^ x[u].z ^ x[u].w;
+      for (int u = 0; u < U; ++u) acc ^= x[u].x ^ x[u].y ^ x[u].z ^ x[u].w;
+    }
   }
   if (acc == 0x9E3779B9u) atomicXor(out, acc);  // keeps the loads live
 }
@@ -135,9 +140,7 @@ extern "C" int rk_probe_read(const void* data, int64_t bytes, void* out4, void* 
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_probe_read, 256, 0) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
-  // slices must stay below 2^32 bytes for the 32-bit buffer offsets
-  int64_t grid = (int64_t)cus * per_cu;
-  while (bytes / grid > (int64_t(1) << 31)) grid *= 2;
+  const int64_t grid = (int64_t)cus * per_cu;
   hipLaunchKernelGGL(k_probe_read, dim3((unsigned)grid), dim3(256), 0,
                      static_cast<hipStream_t>(stream), static_cast<const uint4*>(data), bytes / 16,
                      static_cast<unsigned int*>(out4));

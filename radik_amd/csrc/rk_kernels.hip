@@ -49,18 +49,13 @@ namespace rk {
 constexpr int SEL_NT = 512;          // threads of the per-row select kernels
 constexpr int LDS_KEYS_MAX = 16384;  // compile-time bound of the LDS key buffer
 constexpr int STREAM_NT = 256;       // threads of the streaming kernels
-// The streaming filter reads each key exactly once: non-temporal loads keep
-// it from evicting the row metadata and candidate buffers from the caches.
-#ifndef RK_FILTER_NT
-#define RK_FILTER_NT 1
-#endif
 
 // ---------------------------------------------------------------------------
-// Work partition of the streaming kernels: the tiles of all rows form one
-// flat list (tile_start is its per-row prefix).  Workgroup w takes the
-// contiguous range [T*w/G, T*(w+1)/G), so its tiles of one row form one
-// contiguous element range, and the partition is identical in every launch
-// with the same grid (each tile meets the same XCD's L2 in every pass).
+// Work partition of the exact-pass and ordered-emit kernels: the tiles of
+// all rows form one flat list (tile_start is its per-row prefix).
+// Workgroup w takes the contiguous range [T*w/G, T*(w+1)/G), so its tiles of
+// one row form one contiguous element range (one merge of its histogram per
+// row), and the partition is identical in every launch with the same grid.
 // ---------------------------------------------------------------------------
 template <typename F>
 RK_DEV void for_each_segment(const KParams& p, F&& f) {
@@ -87,6 +82,46 @@ RK_DEV void for_each_segment(const KParams& p, F&& f) {
     f(r, t - rs, se - rs);
     t = se;
     ++r;
+  }
+}
+
+// The filter's partition of the same flat tile list: chunks of chunk_tiles
+// tiles taken from a queue (one atomic per chunk).  Workgroups that stream
+// faster take more chunks, so all finish within about one chunk of each
+// other (a static slice per workgroup measured up to 15% slower on MI355X,
+// the slowest slice setting the kernel's end).  chunk_row[c] (from
+// plan_scan) is the row holding chunk c's first tile.  The next chunk is
+// claimed while the current one is read.
+template <typename F>
+RK_DEV void for_each_chunk_queued(const KParams& p, uint32_t* q, F&& f) {
+  __shared__ uint32_t next_sh;
+  const uint32_t T = p.tile_start[p.R];
+  const uint32_t CT = (uint32_t)p.chunk_tiles;
+  const uint32_t nch = (T + CT - 1) / CT;
+  if (threadIdx.x == 0) next_sh = atomicAdd(q, 1u);
+  __syncthreads();
+  uint32_t c = next_sh;
+  __syncthreads();
+  while (c < nch) {
+    if (threadIdx.x == 0) next_sh = atomicAdd(q, 1u);
+    const uint32_t t0 = c * CT;
+    const uint32_t t1 = (T - t0) < CT ? T : t0 + CT;
+    int r = (int)p.chunk_row[c];
+    uint32_t t = t0;
+    while (t < t1 && r < p.R) {
+      const uint32_t rs = p.tile_start[r], re = p.tile_start[r + 1];
+      if (re <= t) {
+        ++r;
+        continue;
+      }
+      const uint32_t se = re < t1 ? re : t1;
+      f(r, t - rs, se - rs);
+      t = se;
+      ++r;
+    }
+    __syncthreads();
+    c = next_sh;
+    __syncthreads();
   }
 }
 
@@ -244,6 +279,8 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
     h->q_select[0] = 0;
     h->q_select[1] = 0;
     h->sel_done = 0;
+    h->q_chunk[0] = 0;
+    h->q_chunk[1] = 0;
     if (carry_t > p.tile_capacity) status |= ST_TILES;
     if (carry_c > p.cand_capacity) status |= ST_CANDIDATES;
     h->status = status;
@@ -283,6 +320,14 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
       if (ph == PH_SAMPLE || ph == PH_HIST) p.st[r].phase = PH_DONE;
       p.tile_start[r] = 0;
     }
+    return;
+  }
+  // First row of every chunk of the filter's interleaved partition: row r
+  // owns the chunks whose first tile lies in [tile_start[r], tile_start[r+1]).
+  const uint32_t CT = (uint32_t)p.chunk_tiles;
+  for (int r = tid; r < p.R; r += 1024) {
+    const uint32_t ts = p.tile_start[r], te = p.tile_start[r + 1];
+    for (uint32_t c = (ts + CT - 1) / CT; c * CT < te; ++c) p.chunk_row[c] = (uint32_t)r;
   }
 }
 
@@ -446,22 +491,31 @@ __global__ __launch_bounds__(STREAM_NT) void k_filter(KParams p) {
   // control flow stays scalar and its loads stay unconditional
   const int wave_base = __builtin_amdgcn_readfirstlane(threadIdx.x & ~63);
 
-  for_each_segment(p, [&](int r, uint32_t ta, uint32_t tb) {
-    if (p.st[r].phase != PH_FILTER) return;  // uniform across the block
+  auto body = [&](int r, uint32_t ta, uint32_t tb) {
+    // the row's metadata, loaded together (one round trip)
+    const RowState st = p.st[r];
     const RowDesc d = p.desc[r];
-    const uint32_t T = p.st[r].T;
-    const uint32_t cap = p.st[r].cap;
     const uint64_t cbase = p.cand_base[r];
+    if (st.phase != PH_FILTER) return;  // uniform across the block
+    const uint32_t T = st.T;
+    const uint32_t cap = st.cap;
     const bool all = (T == 0);
     const float tf = all ? 0.0f : key_threshold_float<DT, LARGEST>(T);
     const UT* rb = static_cast<const UT*>(d.base);
-    const int64_t es = (int64_t)ta * p.tune.tile;
-    int64_t ee = (int64_t)tb * p.tune.tile;
-    if (ee > d.len) ee = d.len;
-    int64_t al = aligned_start<DT>(rb, es);
-    if (al > ee) al = ee;
-    const int64_t nv = (ee - al) / P;
-    const int64_t tail0 = al + nv * P;
+    // 16-byte vectors are counted from the row's first aligned element a0;
+    // the segment owns vectors [ta * tile / P, tb * tile / P) of that grid,
+    // so only the row's first segment has head elements [0, a0) and only its
+    // last segment the tail elements after the last whole vector.
+    const int64_t a0 = aligned_start<DT>(rb, 0) < d.len ? aligned_start<DT>(rb, 0) : d.len;
+    const int64_t nv_row = (d.len - a0) / P;
+    const int64_t v_lo = (int64_t)ta * (p.tune.tile / P);
+    int64_t v_hi = (int64_t)tb * (p.tune.tile / P);
+    if (v_hi > nv_row) v_hi = nv_row;
+    const int64_t al = a0 + (v_lo < v_hi ? v_lo : v_hi) * P;  // first element of the body
+    const int64_t nv = v_hi > v_lo ? v_hi - v_lo : 0;
+    const bool first_seg = (ta == 0);
+    const bool last_seg = (int64_t)tb * p.tune.tile >= d.len;
+    const int64_t tail0 = a0 + nv_row * P;
 
     auto pred = [&](uint32_t u) -> bool {
       if (all) return true;
@@ -475,11 +529,13 @@ __global__ __launch_bounds__(STREAM_NT) void k_filter(KParams p) {
     ac.cap = cap;
     uint32_t staged = 0;  // keys in this wave's stage (wave-uniform)
 
-    // Head and tail elements outside the aligned body: wave 0, one per lane.
-    if (wave_base == 0) {
+    // The row's head and tail elements outside the vector grid: wave 0 of
+    // the first / last segment, one per lane.
+    if (wave_base == 0 && (first_seg || last_seg)) {
       int64_t pos = -1;
-      if (lane < P && es + lane < al) pos = es + lane;
-      if (lane >= 32 && lane - 32 < P && tail0 + (lane - 32) < ee) pos = tail0 + (lane - 32);
+      if (first_seg && lane < P && lane < a0) pos = lane;
+      if (last_seg && lane >= 32 && lane - 32 < P && tail0 + (lane - 32) < d.len)
+        pos = tail0 + (lane - 32);
       uint32_t mask = 0;
       uint4 x = make_uint4(0u, 0u, 0u, 0u);
       if (pos >= 0) {
@@ -513,18 +569,19 @@ __global__ __launch_bounds__(STREAM_NT) void k_filter(KParams p) {
         const __amdgpu_buffer_rsrc_t rs = make_rsrc(vp + c0, nvc * 16u);
         auto load_set = [&](uint4 (&x)[U], uint32_t jb) {
 #pragma unroll
-          for (int u = 0; u < U; ++u) x[u] = bload16(rs, (jb + lane + u * STREAM_NT) * 16u);
+          for (int u = 0; u < U; ++u) x[u] = bload16_nt(rs, (jb + lane + u * STREAM_NT) * 16u);
         };
         auto test_set = [&](const uint4 (&x)[U], uint32_t jb) {
-          bool hit = false;
+          uint32_t hb = 0;  // bit u: this lane's vector u holds a candidate
 #pragma unroll
           for (int u = 0; u < U; ++u) {
             const float b = vec_best<DT, LARGEST>(x[u]);
-            hit |= LARGEST ? (b >= tf) : (b <= tf);
+            hb |= ((LARGEST ? (b >= tf) : (b <= tf)) ? 1u : 0u) << u;
           }
-          if (__ballot(hit)) {  // rare: exact per-element masks
-#pragma unroll
+          if (__ballot(hb != 0)) {  // rare: exact per-element masks, only for
+#pragma unroll                      // the vector slots some lane hit
             for (int u = 0; u < U; ++u) {
+              if (!__ballot((hb >> u) & 1u)) continue;
               const uint32_t j = jb + lane + u * STREAM_NT;
               const Vec<DT> v = unpack<DT>(x[u]);
               uint32_t mm = 0;
@@ -533,7 +590,7 @@ __global__ __launch_bounds__(STREAM_NT) void k_filter(KParams p) {
                 const float xf = bits_to_float<DT>(v.e[c]);
                 mm |= ((LARGEST ? (xf >= tf) : (xf <= tf)) ? 1u : 0u) << c;
               }
-              mm = j < nvc ? mm : 0u;
+              mm = (j < nvc && ((hb >> u) & 1u)) ? mm : 0u;
               if (__ballot(mm != 0)) append_vec<DT, LARGEST>(ac, staged, x[u], mm, al + (c0 + j) * P);
             }
           }
@@ -549,7 +606,8 @@ __global__ __launch_bounds__(STREAM_NT) void k_filter(KParams p) {
       }
     }
     wave_flush(ac, staged);
-  });
+  };
+  for_each_chunk_queued(p, &p.hdr->q_chunk[p.round], body);
 }
 
 // ---------------------------------------------------------------------------

@@ -71,7 +71,8 @@ struct Header {
   uint32_t q_sample;       // row queue heads of the per-row kernels
   uint32_t q_select[2];    //   (select: sampled round, fallback round)
   uint32_t sel_done;       // workgroups of the sampled-round select that finished
-  uint32_t pad[54];
+  uint32_t q_chunk[2];     // chunk queue heads of the filter (sampled, fallback round)
+  uint32_t pad[52];
 };
 static_assert(sizeof(Header) == 256, "workspace header is 256 bytes");
 
@@ -114,6 +115,7 @@ struct KParams {
   uint32_t* ghist;       // [R][2048]
   uint2* tile_cnt;       // [tile_capacity]
   uint2* tile_pre;       // [tile_capacity]
+  uint32_t* chunk_row;   // [tile_capacity] first row of each chunk (filter)
   uint64_t* cand;        // [cand_capacity]
   uint64_t tile_capacity;
   uint64_t cand_capacity;
@@ -121,6 +123,7 @@ struct KParams {
   int32_t small_max;    // rows up to this length select straight from the input
   int32_t sample_keys;  // LDS key capacity of the sample kernel (this launch)
   int32_t round;        // 0: sampled round; 1: fallback round (skips when idle)
+  int32_t chunk_tiles;  // tiles per chunk of the interleaved filter partition
   Tune tune;
   // When set (sampled-round select only): the last select workgroup to finish
   // copies the header's first 8 words here (mapped host memory).
