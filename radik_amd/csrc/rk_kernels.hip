@@ -85,28 +85,59 @@ RK_DEV void for_each_segment(const KParams& p, F&& f) {
   }
 }
 
+// Row holding tile t: the last r in [0, R) with tile_start[r] <= t (rows
+// without tiles never qualify for t < tile_start[R]).  A 64-way search by
+// one whole wave: each step loads 64 evenly spaced prefix entries and keeps
+// the span between the last one <= t and the next, so R rows take
+// ceil(log64 R) dependent loads (2 for 2048 rows).
+RK_DEV int find_row_wave(const uint32_t* tile_start, int R, uint32_t t) {
+  const int lane = lane_id();
+  int lo = 0, hi = R;
+  while (hi - lo > 1) {
+    const int64_t span = hi - lo;
+    const int pos = lo + (int)((span * lane) / 64);
+    const uint64_t m = __ballot(ldg(tile_start + pos) <= t);  // a prefix of the lanes
+    const int last = 63 - __builtin_clzll(m);
+    const int nlo = lo + (int)((span * last) / 64);
+    hi = last == 63 ? hi : lo + (int)((span * (last + 1)) / 64);
+    lo = nlo;
+  }
+  return lo;
+}
+
 // The filter's partition of the same flat tile list: chunks of chunk_tiles
 // tiles taken from a queue (one atomic per chunk).  Workgroups that stream
 // faster take more chunks, so all finish within about one chunk of each
 // other (a static slice per workgroup measured up to 15% slower on MI355X,
-// the slowest slice setting the kernel's end).  chunk_row[c] (from
-// plan_scan) is the row holding chunk c's first tile.  The next chunk is
-// claimed while the current one is read.
+// the slowest slice setting the kernel's end).  The next chunk is claimed
+// while the current one is read.  When there are no more chunks than
+// workgroups (small calls), chunk = workgroup index, without the queue.
 template <typename F>
 RK_DEV void for_each_chunk_queued(const KParams& p, uint32_t* q, F&& f) {
   __shared__ uint32_t next_sh;
+  __shared__ int row_sh;
   const uint32_t T = p.tile_start[p.R];
   const uint32_t CT = (uint32_t)p.chunk_tiles;
   const uint32_t nch = (T + CT - 1) / CT;
-  if (threadIdx.x == 0) next_sh = atomicAdd(q, 1u);
-  __syncthreads();
-  uint32_t c = next_sh;
+  const bool direct = nch <= gridDim.x;
+  const bool wave0 = threadIdx.x < 64;
+  uint32_t c = blockIdx.x;
+  if (!direct) {
+    if (threadIdx.x == 0) next_sh = atomicAdd(q, 1u);
+    __syncthreads();
+    c = next_sh;
+  }
+  if (c >= nch) return;  // uniform: c is the same for the whole workgroup
+  if (wave0) {
+    const int r0 = find_row_wave(p.tile_start, p.R, c * CT);
+    if (threadIdx.x == 0) row_sh = r0;
+  }
   __syncthreads();
   while (c < nch) {
-    if (threadIdx.x == 0) next_sh = atomicAdd(q, 1u);
+    int r = row_sh;
+    if (!direct && threadIdx.x == 0) next_sh = atomicAdd(q, 1u);
     const uint32_t t0 = c * CT;
     const uint32_t t1 = (T - t0) < CT ? T : t0 + CT;
-    int r = (int)p.chunk_row[c];
     uint32_t t = t0;
     while (t < t1 && r < p.R) {
       const uint32_t rs = p.tile_start[r], re = p.tile_start[r + 1];
@@ -119,8 +150,13 @@ RK_DEV void for_each_chunk_queued(const KParams& p, uint32_t* q, F&& f) {
       t = se;
       ++r;
     }
+    if (direct) break;
     __syncthreads();
     c = next_sh;
+    if (c < nch && wave0) {
+      const int rn = find_row_wave(p.tile_start, p.R, c * CT);
+      if (threadIdx.x == 0) row_sh = rn;
+    }
     __syncthreads();
   }
 }
@@ -320,14 +356,6 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
       if (ph == PH_SAMPLE || ph == PH_HIST) p.st[r].phase = PH_DONE;
       p.tile_start[r] = 0;
     }
-    return;
-  }
-  // First row of every chunk of the filter's interleaved partition: row r
-  // owns the chunks whose first tile lies in [tile_start[r], tile_start[r+1]).
-  const uint32_t CT = (uint32_t)p.chunk_tiles;
-  for (int r = tid; r < p.R; r += 1024) {
-    const uint32_t ts = p.tile_start[r], te = p.tile_start[r + 1];
-    for (uint32_t c = (ts + CT - 1) / CT; c * CT < te; ++c) p.chunk_row[c] = (uint32_t)r;
   }
 }
 
