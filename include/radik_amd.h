@@ -45,13 +45,16 @@ enum {
   RK_OK = 0,
   RK_E_ARG = -1,         /* bad argument (null pointer, negative size, ...) */
   RK_E_WORKSPACE = -2,   /* workspace missing or smaller than required */
-  RK_E_UNSUPPORTED = -3, /* k above rk_max_k(), or dtype unknown */
+  RK_E_UNSUPPORTED = -3, /* sorted output with k above rk_max_k(), or dtype unknown */
   RK_E_ALIGN = -4,       /* value pointer not aligned to its element size */
   RK_E_HIP = -5,         /* a HIP runtime call failed */
   RK_E_DEVICE = -6       /* device-side validation failed (rk_topk_status) */
 };
 
-/* Largest k any entry point accepts on the current device (16384 on MI355X). */
+/* Largest k for sorted output on the current device (16384 on MI355X: the
+ * LDS sort).  Unsorted calls accept any k: rows whose k exceeds this limit
+ * (e.g. a median, k = len / 2) go through the exact digit passes and an
+ * ordered emit in position order, with no candidate buffer. */
 int32_t rk_max_k(void);
 
 /* Workspace bytes for a batched call over at most `num_rows` rows with at

@@ -44,7 +44,7 @@ _lib = None
 
 _ERRORS = {
     0: "ok", -1: "bad argument", -2: "workspace missing or too small",
-    -3: "unsupported dtype or k above rk_max_k()", -4: "misaligned pointer",
+    -3: "unsupported dtype, or sorted output with k above rk_max_k()", -4: "misaligned pointer",
     -5: "HIP runtime error", -6: "device-side validation failed",
 }
 
@@ -194,6 +194,7 @@ def _flags(largest: bool, sorted: bool, ascend: bool | None, exact: bool) -> int
 
 
 def max_k() -> int:
+    """Largest k for sorted output (unsorted output accepts any k)."""
     return int(lib().rk_max_k())
 
 

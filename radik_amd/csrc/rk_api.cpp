@@ -73,7 +73,9 @@ Layout make_layout(int32_t R, int64_t total_len, int32_t max_k) {
   const uint64_t Ru = (uint64_t)R;
   const uint64_t E = (uint64_t)total_len;
   L.tile_cap = E / rk::TILE_MIN + Ru + 1;
-  const uint64_t cand_bound = 4 * Ru * (uint64_t)max_k + E / 64 + 1025 * Ru;
+  // rows with k beyond the LDS select take no candidate slots
+  const uint64_t kc = (uint64_t)std::min(max_k, rk::lds_keys_max());
+  const uint64_t cand_bound = 4 * Ru * kc + E / 64 + 1025 * Ru;
   L.cand_cap = std::min<uint64_t>(E, cand_bound);
   size_t o = 0;
   L.hdr = o; o = align256(o + sizeof(rk::Header));
@@ -143,7 +145,8 @@ int launch(int dtype, const void* vals, const int64_t* row_off, const void* cons
   if (row_ptrs ? !row_len : (!row_off || (!vals && total_len > 0))) return RK_E_ARG;
   const DevInfo di = device_info();
   if (!di.ok) return RK_E_HIP;
-  if (max_k > di.lds_keys) return RK_E_UNSUPPORTED;
+  // k beyond the LDS select: unsorted output only (exact passes + ordered emit)
+  if (max_k > di.lds_keys && (flags & RK_SORTED)) return RK_E_UNSUPPORTED;
   const int es = elem_size(dtype);
   if (((uintptr_t)vals % es) || ((uintptr_t)out_vals % es)) return RK_E_ALIGN;
   if (!workspace) return RK_E_WORKSPACE;
@@ -186,7 +189,7 @@ int launch(int dtype, const void* vals, const int64_t* row_off, const void* cons
   // rows up to that length) live in LDS; larger candidate sets are selected
   // from global memory.  Smaller buffers let more workgroups share a CU.
   int lk = 4096;
-  while (lk < max_k) lk <<= 1;
+  while (lk < max_k && lk < di.lds_keys) lk <<= 1;
   p.lds_keys = std::min(lk, di.lds_keys);
   p.small_max = p.lds_keys;
   p.tune = current_tune(di, dtype, flags, num_rows, total_len, max_k);
@@ -347,7 +350,7 @@ const char* rk_strerror(int code) {
     case RK_OK: return "ok";
     case RK_E_ARG: return "bad argument";
     case RK_E_WORKSPACE: return "workspace missing or too small";
-    case RK_E_UNSUPPORTED: return "unsupported dtype or k above rk_max_k()";
+    case RK_E_UNSUPPORTED: return "unsupported dtype, or sorted output with k above rk_max_k()";
     case RK_E_ALIGN: return "pointer not aligned to its element size (workspace: 256 bytes)";
     case RK_E_HIP: return "HIP runtime error";
     case RK_E_DEVICE: return "device-side validation failed";

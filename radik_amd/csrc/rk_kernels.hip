@@ -222,10 +222,14 @@ RK_DEV void plan_row(const KParams& p, int r) {
   } else if (len <= p.small_max) {
     s.phase = PH_SMALL;
   } else {
-    s.phase = (p.flags & FL_EXACT) ? PH_HIST : PH_SAMPLE;
+    // k beyond the LDS select (unsorted output only; the host rejects sorted
+    // calls with such k): exact passes, then the ordered emit, which needs
+    // no candidate slots and writes position order directly.
+    const bool bigk = k > p.lds_keys;
+    s.phase = (bigk || (p.flags & FL_EXACT)) ? PH_HIST : PH_SAMPLE;
     if (s.phase == PH_HIST) s.flags = RF_NEED_INIT;
     tiles = (uint32_t)((len + p.tune.tile - 1) / p.tune.tile);
-    const int64_t cap = 4 * k + (len + 63) / 64 + 1024;
+    const int64_t cap = bigk ? 0 : 4 * k + (len + 63) / 64 + 1024;
     s.cap = (uint32_t)(cap < len ? cap : len);
     // A row whose slots hold all of it gains nothing from a threshold: every
     // key becomes a candidate (T = 0) and the select works on the whole row.
@@ -735,7 +739,19 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
     const U* rb = static_cast<const U*>(d.base);
     const int64_t len = d.len;
     const uint32_t k = (uint32_t)d.k;
-    if (ph == PH_COPYALL) {
+    if (ph == PH_COPYALL && mode == ORD_POS) {
+      // position order: the row itself, then the tail fill (any length)
+      const uint32_t m = (uint32_t)(len < (int64_t)k ? len : (int64_t)k);
+      U* ov = static_cast<U*>(d.out_v);
+      for (uint32_t i = threadIdx.x; i < m; i += SEL_NT) {
+        ov[i] = ldg(rb + i);
+        d.out_i[i] = d.idx_in ? d.idx_in[i] : (int32_t)i;
+      }
+      for (int64_t i = (int64_t)m + threadIdx.x; i < d.slots; i += SEL_NT) {
+        ov[i] = (U)0;
+        d.out_i[i] = -1;
+      }
+    } else if (ph == PH_COPYALL) {
       const uint32_t m = (uint32_t)(len < (int64_t)k ? len : (int64_t)k);
       for (uint32_t i = threadIdx.x; i < m; i += SEL_NT)
         sbuf[i] = order_key<DT>(mode, sel_key<DT, LARGEST>((uint32_t)ldg(rb + i)), i);
@@ -1192,12 +1208,25 @@ __global__ __launch_bounds__(SEL_NT) void k_finalize_ordered(KParams p) {
     if (p.st[r].phase != PH_ORDERED) continue;
     const RowDesc d = p.desc[r];
     const uint32_t m = (uint32_t)d.k;
-    const U* ov = static_cast<const U*>(d.out_v);
-    const int32_t* oi = d.out_i;
-    for (uint32_t i = threadIdx.x; i < m; i += SEL_NT)
-      sbuf[i] = order_key<DT>(mode, sel_key<DT, LARGEST>((uint32_t)ov[i]), (uint32_t)oi[i]);
-    __syncthreads();
-    sort_and_write<DT, LARGEST>(p, d, sbuf, m, mode);
+    if (mode == ORD_POS) {
+      // the emit wrote values and positions in position order already: map
+      // positions through idx_in and write the tail fill (any k, no LDS)
+      U* ow = static_cast<U*>(d.out_v);
+      int32_t* iw = d.out_i;
+      if (d.idx_in)
+        for (uint32_t i = threadIdx.x; i < m; i += SEL_NT) iw[i] = d.idx_in[iw[i]];
+      for (int64_t i = (int64_t)m + threadIdx.x; i < d.slots; i += SEL_NT) {
+        ow[i] = (U)0;
+        iw[i] = -1;
+      }
+    } else {
+      const U* ov = static_cast<const U*>(d.out_v);
+      const int32_t* oi = d.out_i;
+      for (uint32_t i = threadIdx.x; i < m; i += SEL_NT)
+        sbuf[i] = order_key<DT>(mode, sel_key<DT, LARGEST>((uint32_t)ov[i]), (uint32_t)oi[i]);
+      __syncthreads();
+      sort_and_write<DT, LARGEST>(p, d, sbuf, m, mode);
+    }
     __syncthreads();
     if (threadIdx.x == 0) p.st[r].phase = PH_DONE;
     __syncthreads();

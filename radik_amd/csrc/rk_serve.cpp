@@ -371,7 +371,7 @@ int rk_server_create(const rk_server_config* cfg, rk_server** out) {
   if (cfg->max_batch_rows <= 0 || cfg->max_batch_keys <= 0 || cfg->streams <= 0 ||
       cfg->mem_budget <= 0 || cfg->max_k <= 0 || (cfg->flags & RK_EXACT))
     return RK_E_ARG;
-  if (cfg->max_k > rk_max_k()) return RK_E_UNSUPPORTED;
+  if (cfg->max_k > rk_max_k() && (cfg->flags & RK_SORTED)) return RK_E_UNSUPPORTED;
   rk_server* s = new rk_server;
   s->cfg = *cfg;
   if (s->cfg.plain) {  // the baseline: one request per call, one stream

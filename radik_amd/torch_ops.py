@@ -53,8 +53,9 @@ def topk(values: torch.Tensor, k: int, dim: int = -1, largest: bool = True,
     n = values.shape[d] if values.dim() else 1
     if k < 0 or k > n:
         raise ValueError(f"radik_amd::topk: k={k} out of range for a dimension of size {n}")
-    if k > _max_k():
-        raise ValueError(f"radik_amd::topk: k={k} above the supported maximum {_max_k()}")
+    if sorted and k > _max_k():
+        raise ValueError(f"radik_amd::topk: sorted output supports k up to {_max_k()} "
+                         f"(k={k}); unsorted output has no limit")
     x = values.movedim(d, -1).contiguous() if values.dim() else values.reshape(1)
     lead = x.shape[:-1]
     rows = x.numel() // n if n else 0
@@ -93,8 +94,9 @@ def topk_ragged(values: torch.Tensor, offsets: torch.Tensor, ks: torch.Tensor, m
     if offsets.dtype != torch.int64 or ks.dtype != torch.int32 or \
             offsets.numel() != ks.numel() + 1:
         raise TypeError("radik_amd::topk_ragged: offsets int64 [R+1] and ks int32 [R] expected")
-    if max_k < 0 or max_k > _max_k():
-        raise ValueError(f"radik_amd::topk_ragged: max_k={max_k} out of range")
+    if max_k < 0 or (sorted and max_k > _max_k()):
+        raise ValueError(f"radik_amd::topk_ragged: max_k={max_k} out of range for "
+                         f"{'sorted' if sorted else 'unsorted'} output")
     rows = ks.numel()
     out_v = torch.empty((rows, max_k), dtype=values.dtype, device=values.device)
     out_i = torch.empty((rows, max_k), dtype=torch.int32, device=values.device)
