@@ -201,11 +201,12 @@ int launch(int dtype, const void* vals, const int64_t* row_off, const void* cons
   const uint64_t tiles_bound = (uint64_t)total_len / (uint64_t)p.tune.tile + (uint64_t)num_rows;
   const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)p.tune.grid, tiles_bound));
   // The filter's workgroups take chunks of tiles from a queue.  Chunks of
-  // about total / (8 x grid) bytes, within [256 KB, 2 MB]: large enough to
+  // about total / (8 x grid) bytes, within [64 KB, 2 MB]: large enough to
   // amortise a chunk's start-up latency, small enough that all workgroups
-  // finish within about one chunk of each other.  Small calls use smaller
-  // chunks so that every workgroup still gets two.
-  int64_t chunk_bytes = int64_t(256) << 10;
+  // finish within about one chunk of each other (the last quarter of the
+  // tiles goes in quarter-size chunks).  Small calls use smaller chunks so
+  // that every workgroup still gets two.
+  int64_t chunk_bytes = int64_t(64) << 10;
   while (chunk_bytes < (int64_t(2) << 20) && chunk_bytes * 8 * grid < total_len * es)
     chunk_bytes <<= 1;
   int64_t ct = std::max<int64_t>(1, chunk_bytes / ((int64_t)p.tune.tile * es));

@@ -50,6 +50,7 @@ constexpr int SEL_NT = 512;          // threads of the per-row select kernels
 constexpr int LDS_KEYS_MAX = 16384;  // compile-time bound of the LDS key buffer
 constexpr int STREAM_NT = 256;       // threads of the streaming kernels
 
+
 // ---------------------------------------------------------------------------
 // Work partition of the exact-pass and ordered-emit kernels: the tiles of
 // all rows form one flat list (tile_start is its per-row prefix).
@@ -118,7 +119,13 @@ RK_DEV void for_each_chunk_queued(const KParams& p, uint32_t* q, F&& f) {
   __shared__ int row_sh;
   const uint32_t T = p.tile_start[p.R];
   const uint32_t CT = (uint32_t)p.chunk_tiles;
-  const uint32_t nch = (T + CT - 1) / CT;
+  // Guided sizes: the first ~3/4 of the tiles in chunks of CT, the rest in
+  // chunks of CT/4, so that the last chunks (whose end sets the kernel's)
+  // are short.
+  const uint32_t CS = CT >= 4 ? CT / 4 : CT;
+  const uint32_t n_big = CS < CT ? (uint32_t)(((uint64_t)T * 3 / 4) / CT) : (T + CT - 1) / CT;
+  const uint32_t t_s = n_big * CT < T ? n_big * CT : T;
+  const uint32_t nch = n_big + (T - t_s + CS - 1) / CS;
   const bool direct = nch <= gridDim.x;
   const bool wave0 = threadIdx.x < 64;
   uint32_t c = blockIdx.x;
@@ -128,16 +135,18 @@ RK_DEV void for_each_chunk_queued(const KParams& p, uint32_t* q, F&& f) {
     c = next_sh;
   }
   if (c >= nch) return;  // uniform: c is the same for the whole workgroup
+  auto chunk_start = [&](uint32_t cc) { return cc < n_big ? cc * CT : t_s + (cc - n_big) * CS; };
   if (wave0) {
-    const int r0 = find_row_wave(p.tile_start, p.R, c * CT);
+    const int r0 = find_row_wave(p.tile_start, p.R, chunk_start(c));
     if (threadIdx.x == 0) row_sh = r0;
   }
   __syncthreads();
   while (c < nch) {
     int r = row_sh;
     if (!direct && threadIdx.x == 0) next_sh = atomicAdd(q, 1u);
-    const uint32_t t0 = c * CT;
-    const uint32_t t1 = (T - t0) < CT ? T : t0 + CT;
+    const uint32_t t0 = chunk_start(c);
+    const uint32_t len_c = c < n_big ? CT : CS;
+    const uint32_t t1 = (T - t0) < len_c ? T : t0 + len_c;
     uint32_t t = t0;
     while (t < t1 && r < p.R) {
       const uint32_t rs = p.tile_start[r], re = p.tile_start[r + 1];
@@ -154,7 +163,7 @@ RK_DEV void for_each_chunk_queued(const KParams& p, uint32_t* q, F&& f) {
     __syncthreads();
     c = next_sh;
     if (c < nch && wave0) {
-      const int rn = find_row_wave(p.tile_start, p.R, c * CT);
+      const int rn = find_row_wave(p.tile_start, p.R, chunk_start(c));
       if (threadIdx.x == 0) row_sh = rn;
     }
     __syncthreads();
@@ -512,8 +521,12 @@ RK_DEV void append_vec(const AppendCtx& ac, uint32_t& n, uint4 x, uint32_t mask,
   n += tot;
 }
 
+// At most 72 VGPRs (7 waves per SIMD) for 2 and 4 loads in flight: the
+// rarely taken candidate path would otherwise cost the 16-bit variants
+// occupancy (83 VGPRs, 5 waves); the limit causes no spills there.
 template <int DT, bool LARGEST, int U>
-__global__ __launch_bounds__(STREAM_NT) void k_filter(KParams p) {
+__global__ __launch_bounds__(STREAM_NT) __attribute__((amdgpu_waves_per_eu(U >= 8 ? 1 : 7)))
+void k_filter(KParams p) {
   if (p.round && p.hdr->fallback_rows == 0) return;  // nothing fell back
   using UT = typename Traits<DT>::U;
   constexpr int P = Traits<DT>::PACK;
