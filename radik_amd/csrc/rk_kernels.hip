@@ -696,12 +696,15 @@ RK_DEV void order_decode(int mode, uint64_t key, uint32_t& sk, uint32_t& pos) {
 
 // Sort sbuf[0, m) (re-keyed) and write the row's output plus its tail fill.
 template <int DT, bool LARGEST>
-RK_DEV void sort_and_write(const KParams& p, const RowDesc& d, uint64_t* sbuf, uint32_t m, int mode) {
+RK_DEV void sort_and_write(const KParams& p, const RowDesc& d, uint64_t* sbuf, uint32_t m, int mode,
+                           int r = 0) {
   using U = typename Traits<DT>::U;
   const uint32_t P2 = next_pow2(m);
   for (uint32_t i = m + threadIdx.x; i < P2; i += SEL_NT) sbuf[i] = 0;
   __syncthreads();
   if (P2 > 1) block_bitonic_desc<SEL_NT>(sbuf, P2);
+  RK_STAMP(p, r, 13);
+  (void)r;
   const U* rb = static_cast<const U*>(d.base);
   U* ov = static_cast<U*>(d.out_v);
   int32_t* oi = d.out_i;
@@ -857,7 +860,7 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
       __syncthreads();
       if (threadIdx.x == 0 && slot_cnt != k) atomicOr(&p.hdr->status, (uint32_t)ST_INTERNAL);
       RK_STAMP(p, r, 11);
-      sort_and_write<DT, LARGEST>(p, d, sbuf, k, mode);
+      sort_and_write<DT, LARGEST>(p, d, sbuf, k, mode, r);
       RK_STAMP(p, r, 12);
     }
     __syncthreads();
