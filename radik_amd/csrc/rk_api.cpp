@@ -60,8 +60,8 @@ DevInfo device_info() {
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct Layout {
-  size_t hdr, desc, st, tile_start, cand_base, cand_cnt, order, ghist, tile_cnt, tile_pre, cand,
-      total;
+  size_t hdr, desc, st, tile_start, cand_base, cand_cnt, order, ghist, tile_cnt, tile_pre,
+      chunk_row, cand, total;
   uint64_t tile_cap, cand_cap;
 };
 
@@ -88,6 +88,7 @@ Layout make_layout(int32_t R, int64_t total_len, int32_t max_k) {
   L.ghist = o; o = align256(o + Ru * rk::HIST_MAX_BINS * sizeof(uint32_t));
   L.tile_cnt = o; o = align256(o + L.tile_cap * sizeof(uint2));
   L.tile_pre = o; o = align256(o + L.tile_cap * sizeof(uint2));
+  L.chunk_row = o; o = align256(o + L.tile_cap * sizeof(uint32_t));
   L.cand = o; o = align256(o + L.cand_cap * sizeof(uint64_t));
   L.total = o;
   return L;
@@ -182,6 +183,7 @@ int launch(int dtype, const void* vals, const int64_t* row_off, const void* cons
   p.ghist = reinterpret_cast<uint32_t*>(ws + L.ghist);
   p.tile_cnt = reinterpret_cast<uint2*>(ws + L.tile_cnt);
   p.tile_pre = reinterpret_cast<uint2*>(ws + L.tile_pre);
+  p.chunk_row = reinterpret_cast<uint32_t*>(ws + L.chunk_row);
   p.cand = reinterpret_cast<uint64_t*>(ws + L.cand);
   p.tile_capacity = L.tile_cap;
   p.cand_capacity = L.cand_cap;
@@ -201,12 +203,12 @@ int launch(int dtype, const void* vals, const int64_t* row_off, const void* cons
   const uint64_t tiles_bound = (uint64_t)total_len / (uint64_t)p.tune.tile + (uint64_t)num_rows;
   const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)p.tune.grid, tiles_bound));
   // The filter's workgroups take chunks of tiles from a queue.  Chunks of
-  // about total / (8 x grid) bytes, within [64 KB, 2 MB]: large enough to
+  // about total / (8 x grid) bytes, within [256 KB, 2 MB]: large enough to
   // amortise a chunk's start-up latency, small enough that all workgroups
   // finish within about one chunk of each other (the last quarter of the
   // tiles goes in quarter-size chunks).  Small calls use smaller chunks so
   // that every workgroup still gets two.
-  int64_t chunk_bytes = int64_t(64) << 10;
+  int64_t chunk_bytes = int64_t(256) << 10;
   while (chunk_bytes < (int64_t(2) << 20) && chunk_bytes * 8 * grid < total_len * es)
     chunk_bytes <<= 1;
   int64_t ct = std::max<int64_t>(1, chunk_bytes / ((int64_t)p.tune.tile * es));

@@ -106,47 +106,83 @@ RK_DEV int find_row_wave(const uint32_t* tile_start, int R, uint32_t t) {
   return lo;
 }
 
-// The filter's partition of the same flat tile list: chunks of chunk_tiles
-// tiles taken from a queue (one atomic per chunk).  Workgroups that stream
-// faster take more chunks, so all finish within about one chunk of each
-// other (a static slice per workgroup measured up to 15% slower on MI355X,
-// the slowest slice setting the kernel's end).  The next chunk is claimed
-// while the current one is read.  When there are no more chunks than
-// workgroups (small calls), chunk = workgroup index, without the queue.
+// The filter's partition of the same flat tile list into chunks of
+// chunk_tiles tiles, guided: the first ~3/4 of the tiles in chunks of CT,
+// the rest in chunks of CT/4, so that the last chunks (whose end sets the
+// kernel's) are short.
+struct ChunkGrid {
+  uint32_t T, CT, CS, n_big, t_s, nch;
+  RK_DEV ChunkGrid(uint32_t T_, uint32_t CT_) : T(T_), CT(CT_) {
+    CS = CT >= 4 ? CT / 4 : CT;
+    n_big = CS < CT ? (uint32_t)(((uint64_t)T * 3 / 4) / CT) : (T + CT - 1) / CT;
+    t_s = n_big * CT < T ? n_big * CT : T;
+    nch = n_big + (T - t_s + CS - 1) / CS;
+  }
+  RK_DEV uint32_t start(uint32_t c) const { return c < n_big ? c * CT : t_s + (c - n_big) * CS; }
+  RK_DEV uint32_t size(uint32_t c) const { return c < n_big ? CT : CS; }
+  // first chunk whose first tile is >= t (t <= T)
+  RK_DEV uint32_t first_ge(uint32_t t) const {
+    return t <= t_s ? (t + CT - 1) / CT : n_big + (t - t_s + CS - 1) / CS;
+  }
+};
+constexpr uint32_t NO_ROW = 0xffffffffu;
+
+// Records row r as the row of every chunk that starts inside it (called by
+// a whole workgroup of NT threads for each sampled row).
+template <int NT>
+RK_DEV void record_chunk_rows(const KParams& p, int r) {
+  const ChunkGrid g(p.tile_start[p.R], (uint32_t)p.chunk_tiles);
+  const uint32_t c0 = g.first_ge(p.tile_start[r]), c1 = g.first_ge(p.tile_start[r + 1]);
+  for (uint32_t c = c0 + threadIdx.x % NT; c < c1; c += NT) p.chunk_row[c] = (uint32_t)r;
+}
+
+// Chunks are taken from a queue (one atomic per chunk): workgroups that
+// stream faster take more of them, so all finish within about one chunk of
+// each other (a static slice per workgroup, or static round-robin chunks,
+// measured up to 15% slower on MI355X, the slowest workgroup setting the
+// kernel's end).  A chunk's start-up would otherwise wait on dependent
+// loads at the memory latency of a saturated chip: a workgroup claims its
+// chunks two ahead and loads the next chunk's row from chunk_row while it
+// reads the current one.  Chunks without a recorded row (rows that took the
+// exact passes without sampling) find it with the 64-way search.  When
+// there are no more chunks than workgroups (small calls), chunk =
+// workgroup index.
 template <typename F>
 RK_DEV void for_each_chunk_queued(const KParams& p, uint32_t* q, F&& f) {
-  __shared__ uint32_t next_sh;
+  __shared__ uint32_t c_sh, cn_sh, rn_sh;
   __shared__ int row_sh;
-  const uint32_t T = p.tile_start[p.R];
-  const uint32_t CT = (uint32_t)p.chunk_tiles;
-  // Guided sizes: the first ~3/4 of the tiles in chunks of CT, the rest in
-  // chunks of CT/4, so that the last chunks (whose end sets the kernel's)
-  // are short.
-  const uint32_t CS = CT >= 4 ? CT / 4 : CT;
-  const uint32_t n_big = CS < CT ? (uint32_t)(((uint64_t)T * 3 / 4) / CT) : (T + CT - 1) / CT;
-  const uint32_t t_s = n_big * CT < T ? n_big * CT : T;
-  const uint32_t nch = n_big + (T - t_s + CS - 1) / CS;
+  const ChunkGrid g(p.tile_start[p.R], (uint32_t)p.chunk_tiles);
+  const uint32_t nch = g.nch;
   const bool direct = nch <= gridDim.x;
   const bool wave0 = threadIdx.x < 64;
-  uint32_t c = blockIdx.x;
+  uint32_t c = blockIdx.x, cn = nch;
   if (!direct) {
-    if (threadIdx.x == 0) next_sh = atomicAdd(q, 1u);
+    if (threadIdx.x == 0) {
+      const uint32_t a = atomicAdd(q, 2u);  // the first two chunks
+      c_sh = a;
+      cn_sh = a + 1;
+    }
     __syncthreads();
-    c = next_sh;
+    c = c_sh;
+    cn = cn_sh;
   }
   if (c >= nch) return;  // uniform: c is the same for the whole workgroup
-  auto chunk_start = [&](uint32_t cc) { return cc < n_big ? cc * CT : t_s + (cc - n_big) * CS; };
   if (wave0) {
-    const int r0 = find_row_wave(p.tile_start, p.R, chunk_start(c));
-    if (threadIdx.x == 0) row_sh = r0;
+    uint32_t r0 = ldg(p.chunk_row + c);
+    if (r0 == NO_ROW) r0 = (uint32_t)find_row_wave(p.tile_start, p.R, g.start(c));
+    if (threadIdx.x == 0) row_sh = (int)r0;
   }
   __syncthreads();
-  while (c < nch) {
+  for (;;) {
     int r = row_sh;
-    if (!direct && threadIdx.x == 0) next_sh = atomicAdd(q, 1u);
-    const uint32_t t0 = chunk_start(c);
-    const uint32_t len_c = c < n_big ? CT : CS;
-    const uint32_t t1 = (T - t0) < len_c ? T : t0 + len_c;
+    uint32_t cnn = nch, rn = NO_ROW;
+    if (!direct && threadIdx.x == 0) {
+      // issued now, consumed after this chunk: they complete while it streams
+      cnn = atomicAdd(q, 1u);
+      if (cn < nch) rn = ldg(p.chunk_row + cn);
+    }
+    const uint32_t t0 = g.start(c);
+    const uint32_t t1 = (g.T - t0) < g.size(c) ? g.T : t0 + g.size(c);
     uint32_t t = t0;
     while (t < t1 && r < p.R) {
       const uint32_t rs = p.tile_start[r], re = p.tile_start[r + 1];
@@ -160,11 +196,19 @@ RK_DEV void for_each_chunk_queued(const KParams& p, uint32_t* q, F&& f) {
       ++r;
     }
     if (direct) break;
+    if (threadIdx.x == 0) {
+      c_sh = cn;
+      cn_sh = cnn;
+      rn_sh = rn;
+    }
     __syncthreads();
-    c = next_sh;
-    if (c < nch && wave0) {
-      const int rn = find_row_wave(p.tile_start, p.R, chunk_start(c));
-      if (threadIdx.x == 0) row_sh = rn;
+    c = c_sh;
+    cn = cn_sh;
+    if (c >= nch) break;  // uniform
+    if (wave0) {
+      uint32_t r1 = rn_sh;
+      if (r1 == NO_ROW) r1 = (uint32_t)find_row_wave(p.tile_start, p.R, g.start(c));
+      if (threadIdx.x == 0) row_sh = (int)r1;
     }
     __syncthreads();
   }
@@ -369,6 +413,17 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
       if (ph == PH_SAMPLE || ph == PH_HIST) p.st[r].phase = PH_DONE;
       p.tile_start[r] = 0;
     }
+    return;
+  }
+  // Chunk -> row table of the filter: cleared, then filled for the rows that
+  // skip sampling (the sample kernel fills it for the sampled rows).
+  const ChunkGrid g(p.tile_start[p.R], (uint32_t)p.chunk_tiles);
+  for (uint32_t c = tid; c < g.nch; c += 1024) p.chunk_row[c] = NO_ROW;
+  __syncthreads();
+  for (int r = tid; r < p.R; r += 1024) {
+    if (p.st[r].phase != PH_FILTER) continue;
+    const uint32_t c0 = g.first_ge(p.tile_start[r]), c1 = g.first_ge(p.tile_start[r + 1]);
+    for (uint32_t c = c0; c < c1; ++c) p.chunk_row[c] = (uint32_t)r;
   }
 }
 
@@ -397,6 +452,7 @@ __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
     if (qi >= (uint32_t)p.R) break;
     const int r = (int)p.order[qi];
     if (p.st[r].phase != PH_SAMPLE) continue;
+    record_chunk_rows<SEL_NT>(p, r);
     RK_STAMP(p, r, 0);
     const RowDesc d = p.desc[r];
     const U* rb = static_cast<const U*>(d.base);
@@ -531,6 +587,7 @@ void k_filter(KParams p) {
   using UT = typename Traits<DT>::U;
   constexpr int P = Traits<DT>::PACK;
   __shared__ uint64_t stage[STREAM_NT / 64][WSTAGE];
+  __shared__ uint32_t flush_n[STREAM_NT / 64], flush_base[STREAM_NT / 64];
   const int lane = lane_id();
   // wave-uniform (readfirstlane tells the compiler), so the streaming loop's
   // control flow stays scalar and its loads stay unconditional
@@ -650,7 +707,25 @@ void k_filter(KParams p) {
         }
       }
     }
-    wave_flush(ac, staged);
+    // End of the segment: one atomic per workgroup reserves the slots of
+    // all four waves' staged keys (per-wave atomics on the row's counter,
+    // from every workgroup reading that row, serialized at the counter).
+    if (lane == 0) flush_n[wave_base >> 6] = staged;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t tot = 0;
+      for (int w = 0; w < STREAM_NT / 64; ++w) tot += flush_n[w];
+      uint32_t b = tot ? atomicAdd(ac.cand_cnt, tot) : 0u;
+      for (int w = 0; w < STREAM_NT / 64; ++w) {
+        flush_base[w] = b;
+        b += flush_n[w];
+      }
+    }
+    __syncthreads();
+    const uint32_t fb = flush_base[wave_base >> 6];
+    for (uint32_t i = lane; i < staged; i += 64)
+      if (fb + i < ac.cap) ac.cand[fb + i] = ac.stage[i];
+    staged = 0;
   };
   for_each_chunk_queued(p, &p.hdr->q_chunk[p.round], body);
 }

@@ -115,6 +115,7 @@ struct KParams {
   uint32_t* ghist;       // [R][2048]
   uint2* tile_cnt;       // [tile_capacity]
   uint2* tile_pre;       // [tile_capacity]
+  uint32_t* chunk_row;   // [tile_capacity] row of each filter chunk, or ~0u
   uint64_t* cand;        // [cand_capacity]
   uint64_t tile_capacity;
   uint64_t cand_capacity;
