@@ -144,11 +144,7 @@ RK_DEV __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {
   void* b = (void*)(((uint64_t)hi << 32) | lo);
   return __builtin_amdgcn_make_buffer_rsrc(b, (short)0, (int)n, 0x00020000);
 }
-RK_DEV uint4 bload16(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
-  const rk_v4u v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)byte_off, 0, 0);
-  return make_uint4(v.x, v.y, v.z, v.w);
-}
-// Non-temporal (streaming) variant for data read once.
+// 16-byte buffer load with the non-temporal (streaming) hint: data read once.
 RK_DEV uint4 bload16_nt(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
   const rk_v4u v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)byte_off, 0, 2);
   return make_uint4(v.x, v.y, v.z, v.w);
