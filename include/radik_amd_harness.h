@@ -32,10 +32,12 @@ int rk_gen_fill(int dtype, int kind, uint64_t seed, uint64_t stream_id, int64_t 
                 void* stream);
 
 /* Streaming-read probe: reads `bytes` (rounded down to 16) from the
- * 16-byte-aligned device buffer `data` once, with a resident persistent grid,
- * and writes nothing but (rarely) one word to `out4`.  Timed around, it gives
- * the achievable HBM read bandwidth of the box. */
-int rk_probe_read(const void* data, int64_t bytes, void* out4, void* stream);
+ * 16-byte-aligned device buffer `data` once, with `blocks_per_cu` persistent
+ * workgroups per CU (0: 3; capped at what is resident), and writes nothing
+ * but (rarely) one word to `out4`.  Timed around, the fastest of a few grids
+ * gives the achievable HBM read bandwidth of the box. */
+int rk_probe_read(const void* data, int64_t bytes, int32_t blocks_per_cu, void* out4,
+                  void* stream);
 
 #ifdef __cplusplus
 }

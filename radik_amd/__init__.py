@@ -131,7 +131,7 @@ def lib():
                  "rk_tune_cache_clear", "rk_tune_cache_size", "rk_graph_begin", "rk_graph_end",
                  "rk_graph_launch", "rk_graph_destroy"):
         getattr(L, name).restype = c.c_int
-    L.rk_probe_read.argtypes = [vp, i64, vp, vp]
+    L.rk_probe_read.argtypes = [vp, i64, i32, vp, vp]
     L.rk_probe_read.restype = c.c_int
     P = c.POINTER
     L.rk_server_create.argtypes = [P(_ServerConfig), P(vp)]
@@ -409,11 +409,12 @@ class _GraphCapture:
         return False
 
 
-def probe_read(t, stream=None) -> None:
-    """Launches the streaming-read probe over tensor ``t`` (harness)."""
+def probe_read(t, blocks_per_cu: int = 0, stream=None) -> None:
+    """Launches the streaming-read probe over tensor ``t`` (harness) with
+    ``blocks_per_cu`` workgroups per CU (0: the default)."""
     torch = _torch()
     out = torch.zeros(1, dtype=torch.int32, device=t.device)
-    _check(lib().rk_probe_read(_ptr(t), t.numel() * t.element_size(), _ptr(out),
+    _check(lib().rk_probe_read(_ptr(t), t.numel() * t.element_size(), blocks_per_cu, _ptr(out),
                                _stream_ptr(stream)), "rk_probe_read")
 
 

@@ -115,12 +115,16 @@ Tune current_tune(const DevInfo& di, int dtype, uint32_t flags, int32_t rows, in
   }
   if (t.tile < rk::TILE_MIN) t.tile = rk::TILE_MIN;
   t.tile = (t.tile / 64) * 64;
-  if (t.grid <= 0) t.grid = di.cus * 8;
+  // Default grid: the sampled path's filter keeps two load sets in flight
+  // per wave and runs best at 4 workgroups per CU; the exact path's
+  // histogram passes at 8.
+  const bool exact = (flags & RK_EXACT) != 0;
+  if (t.grid <= 0) t.grid = di.cus * (exact ? 8 : 4);
   if (t.sample_groups <= 0) t.sample_groups = 1024;
   if (t.group_elems <= 0) t.group_elems = 16;
   t.group_elems = std::min(256, std::max(8, (t.group_elems / 8) * 8));
   if (t.hist_bits < 8 || t.hist_bits > 11) t.hist_bits = 11;
-  if (t.unroll != 2 && t.unroll != 4 && t.unroll != 8) t.unroll = 4;
+  if (t.unroll != 2 && t.unroll != 4 && t.unroll != 8) t.unroll = exact ? 4 : 2;
   return t;
 }
 

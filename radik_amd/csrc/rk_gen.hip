@@ -101,9 +101,11 @@ __global__ void k_gen(int dtype, int kind, uint64_t base, int64_t n, int64_t row
 __global__ __launch_bounds__(256) void k_probe_read(const uint4* p, int64_t nvec,
                                                     unsigned int* out) {
   // Workgroups take 256 KB blocks grid-stride, so the blocks read at any
-  // moment lie close together; non-temporal loads (nothing is re-read).
-  // The fastest read pattern measured on MI355X.
-  constexpr int U = 4;
+  // moment lie close together; non-temporal loads (nothing is re-read); two
+  // load sets in flight per wave (the next set is issued before the current
+  // one is consumed).  The fastest read pattern measured on MI355X, at 2-3
+  // workgroups per CU.
+  constexpr int U = 2;
   constexpr int64_t BLK = 16384;  // vectors per block
   typedef unsigned int v4u __attribute__((ext_vector_type(4)));
   const uint32_t wb = __builtin_amdgcn_readfirstlane(threadIdx.x & ~63u);
@@ -117,13 +119,20 @@ __global__ __launch_bounds__(256) void k_probe_read(const uint4* p, int64_t nvec
     const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
         (void*)(((uint64_t)hi << 32) | lo), (short)0,
         (int)__builtin_amdgcn_readfirstlane(n * 16u), 0x00020000);
+    // offsets past n * 16 bytes (at most 2 x 512 vectors beyond) read zeros
+    v4u x[U], y[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      x[u] = __builtin_amdgcn_raw_buffer_load_b128(r, (int)((wb + lane + u * 256) * 16u), 0, 2);
     for (uint32_t jb = wb; jb < n; jb += 256 * U) {
-      v4u x[U];
 #pragma unroll
       for (int u = 0; u < U; ++u)
-        x[u] = __builtin_amdgcn_raw_buffer_load_b128(r, (int)((jb + lane + u * 256) * 16u), 0, 2);
+        y[u] = __builtin_amdgcn_raw_buffer_load_b128(
+            r, (int)((jb + 256 * U + lane + u * 256) * 16u), 0, 2);
 #pragma unroll
       for (int u = 0; u < U; ++u) acc ^= x[u].x ^ x[u].y ^ x[u].z ^ x[u].w;
+#pragma unroll
+      for (int u = 0; u < U; ++u) x[u] = y[u];
     }
   }
   if (acc == 0x9E3779B9u) atomicXor(out, acc);  // keeps the loads live
@@ -131,8 +140,10 @@ __global__ __launch_bounds__(256) void k_probe_read(const uint4* p, int64_t nvec
 
 }  // namespace
 
-extern "C" int rk_probe_read(const void* data, int64_t bytes, void* out4, void* stream) {
-  if (!data || !out4 || bytes < 0 || ((uintptr_t)data % 16)) return RK_E_ARG;
+extern "C" int rk_probe_read(const void* data, int64_t bytes, int32_t blocks_per_cu, void* out4,
+                             void* stream) {
+  if (!data || !out4 || bytes < 0 || ((uintptr_t)data % 16) || blocks_per_cu < 0)
+    return RK_E_ARG;
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return RK_E_HIP;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
@@ -140,7 +151,8 @@ extern "C" int rk_probe_read(const void* data, int64_t bytes, void* out4, void* 
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_probe_read, 256, 0) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
-  const int64_t grid = (int64_t)cus * per_cu;
+  const int bpc = blocks_per_cu == 0 ? 3 : (blocks_per_cu < per_cu ? blocks_per_cu : per_cu);
+  const int64_t grid = (int64_t)cus * bpc;
   hipLaunchKernelGGL(k_probe_read, dim3((unsigned)grid), dim3(256), 0,
                      static_cast<hipStream_t>(stream), static_cast<const uint4*>(data), bytes / 16,
                      static_cast<unsigned int*>(out4));
