@@ -85,6 +85,29 @@ def test_misaligned_rows(pad):
     _check("bf16", b16, off, ks, SORTED | ASCEND, pad_front=pad)
 
 
+@pytest.mark.parametrize("dt", ["f32", "bf16"])
+@pytest.mark.parametrize("pad", [0, 1, 5])
+def test_row_heads_and_tails_at_every_alignment(dt, pad):
+    # The filter streams 16-byte vectors from each row's first element on a
+    # 128-byte boundary; the elements before it (up to 63) and after the
+    # last whole vector are read one per lane.  64 rows of 30001 elements
+    # start at every residue mod 64 elements, and each row's best keys are
+    # planted in its first 64 and last 8 elements (ties, broken by index).
+    rows, n = 64, 30_001
+    off = np.arange(rows + 1, dtype=np.int64) * n
+    ks = np.full(rows, 72, np.int32)
+    bits = _gen_bits(dt, 1, rows * n, 31 + pad)
+    to_bits = (lambda v: np.float32(v).view(np.uint32)) if dt == "f32" else \
+        (lambda v: np.uint16(np.float32(v).view(np.uint32) >> 16))
+    for flags, head, tail in ((LARGEST | SORTED, 1000.0, 2000.0),
+                              (SORTED | ASCEND, -1000.0, -2000.0)):
+        b = bits.copy()
+        for r in range(rows):
+            b[off[r]:off[r] + 64] = to_bits(head)
+            b[off[r + 1] - 8:off[r + 1]] = to_bits(tail)
+        _check(dt, b, off, ks, flags, pad_front=pad)
+
+
 @pytest.mark.parametrize("exact", [False, True])
 def test_all_equal_long_rows_take_lowest_positions(exact):
     """Every key ties: the ties overflow the candidate slots and the ordered
