@@ -235,6 +235,35 @@ int rk_server_get_stats(rk_server* s, rk_server_stats* out);
 /* Drains, stops the dispatcher and frees the server's buffers. */
 int rk_server_destroy(rk_server* s);
 
+/* ---- Fused k-nearest-neighbour select -----------------------------------
+ * No counterpart in the reference, which selects from values already in
+ * memory: this is its top-k fused into the producer of the values, a
+ * distance GEMM (radik/RadixSelect/topk_radixselect.h:59-70 is the select it
+ * extends).  Scores of nq queries against n database vectors are computed
+ * with MFMA (bf16 or f16 inputs, fp32 accumulation) and filtered against a
+ * per-query threshold inside the same kernel, so the nq x n score matrix is
+ * never written; the candidates are selected exactly (ties: lower database
+ * index).
+ *   queries [nq][d], base [n][d]: row-major, 16-byte aligned, dtype RK_BF16
+ *   or RK_F16, d a multiple of 16 in [16, 256], 1 <= k <= min(n, 16384).
+ *   metric RK_KNN_IP: inner product, largest first; RK_KNN_L2: squared L2
+ *   distance |q|^2 - (2 q.x - |x|^2), smallest first.
+ *   flags: RK_SORTED (best first, ties by index) or 0 (database order).
+ *   out_vals [nq][k] fp32 scores / distances, out_idx [nq][k] int32 indices.
+ * rk_topk_status(workspace) afterwards reports RK_E_DEVICE if a query's
+ * candidates at the safe threshold outgrew its slots (only with very many
+ * scores tied at it); its fallback_rows output counts the queries that
+ * needed the second, safe-threshold pass. */
+enum { RK_KNN_IP = 0, RK_KNN_L2 = 1 };
+int rk_knn_workspace_size(int dtype, int32_t nq, int64_t n, int32_t d, int32_t k, size_t* bytes);
+int rk_knn_topk(int dtype, int metric, const void* queries, const void* base, int32_t nq, int64_t n,
+                int32_t d, int32_t k, uint32_t flags, float* out_vals, int32_t* out_idx,
+                void* workspace, size_t ws_bytes, void* stream);
+/* The scores themselves, [nq][n] fp32, with the same arithmetic as
+ * rk_knn_topk (the unfused path: scores to memory, then rk_topk_batched). */
+int rk_knn_scores(int dtype, int metric, const void* queries, const void* base, int32_t nq,
+                  int64_t n, int32_t d, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,6 +34,7 @@ __all__ = [
 
 RK_F32, RK_F16, RK_BF16 = 0, 1, 2
 RK_LARGEST, RK_SORTED, RK_ASCEND, RK_EXACT = 1, 2, 4, 8
+RK_KNN_IP, RK_KNN_L2 = 0, 1
 GEN_UNIFORM, GEN_NORMAL, GEN_ZIPF, GEN_NEARCONST, GEN_RANGE, GEN_CONST = range(6)
 RK_OK, RK_E_ARG, RK_E_WORKSPACE, RK_E_UNSUPPORTED, RK_E_ALIGN, RK_E_HIP, RK_E_DEVICE = (
     0, -1, -2, -3, -4, -5, -6)
@@ -57,7 +58,7 @@ EXPORTED = (
     "rk_tune_cache_size", "rk_graph_begin", "rk_graph_end", "rk_graph_launch",
     "rk_graph_destroy", "rk_probe_read", "rk_server_create", "rk_server_submit",
     "rk_server_wait", "rk_server_drain", "rk_server_replay", "rk_server_get_stats",
-    "rk_server_destroy",
+    "rk_server_destroy", "rk_knn_workspace_size", "rk_knn_topk", "rk_knn_scores",
 )
 
 
@@ -143,6 +144,12 @@ def lib():
     L.rk_server_destroy.argtypes = [vp]
     for name in ("rk_server_create", "rk_server_submit", "rk_server_wait", "rk_server_drain",
                  "rk_server_replay", "rk_server_get_stats", "rk_server_destroy"):
+        getattr(L, name).restype = c.c_int
+    L.rk_knn_workspace_size.argtypes = [c.c_int, i32, i64, i32, i32, P(sz)]
+    L.rk_knn_topk.argtypes = [c.c_int, c.c_int, vp, vp, i32, i64, i32, i32, u32, vp, vp, vp, sz,
+                              vp]
+    L.rk_knn_scores.argtypes = [c.c_int, c.c_int, vp, vp, i32, i64, i32, vp, vp]
+    for name in ("rk_knn_workspace_size", "rk_knn_topk", "rk_knn_scores"):
         getattr(L, name).restype = c.c_int
     L.rk_timing_enable.argtypes = [c.c_int]
     L.rk_timing_read.argtypes = [c.POINTER(c.c_double), c.POINTER(i32)]
@@ -407,6 +414,56 @@ class _GraphCapture:
             _check(rc, "rk_graph_end")
             self.graph._h = h.value
         return False
+
+
+def knn_workspace_size(dtype: int, nq: int, n: int, d: int, k: int) -> int:
+    out = ctypes.c_size_t(0)
+    _check(lib().rk_knn_workspace_size(dtype, nq, n, d, k, ctypes.byref(out)),
+           "rk_knn_workspace_size")
+    return out.value
+
+
+def knn_topk(queries, base, k: int, *, metric: int = RK_KNN_IP, sorted: bool = True,
+             workspace=None, out_vals=None, out_idx=None, stream=None):
+    """Fused k-nearest-neighbour select: the k best database rows of every
+    query (``queries`` [nq, d], ``base`` [n, d], bf16 or f16 device tensors)
+    by inner product (largest) or squared L2 distance (smallest), without the
+    [nq, n] score matrix in memory.  Returns (values [nq, k] float32, indices
+    [nq, k] int32, workspace)."""
+    torch = _torch()
+    if queries.dim() != 2 or base.dim() != 2 or queries.shape[1] != base.shape[1]:
+        raise ValueError("knn_topk: queries [nq, d] and base [n, d] expected")
+    if queries.dtype != base.dtype:
+        raise ValueError("knn_topk: queries and base must share a dtype")
+    dt = _dtype_code(queries)
+    nq, d = queries.shape
+    n = base.shape[0]
+    if workspace is None:
+        workspace = _alloc_ws(knn_workspace_size(dt, nq, n, d, k), queries.device)
+    if out_vals is None:
+        out_vals = torch.empty((nq, k), dtype=torch.float32, device=queries.device)
+    if out_idx is None:
+        out_idx = torch.empty((nq, k), dtype=torch.int32, device=queries.device)
+    rc = lib().rk_knn_topk(dt, metric, _ptr(queries.contiguous()), _ptr(base.contiguous()), nq, n,
+                           d, k, RK_SORTED if sorted else 0, _ptr(out_vals), _ptr(out_idx),
+                           _ptr(workspace), workspace.numel(), _stream_ptr(stream))
+    _check(rc, "rk_knn_topk")
+    return out_vals, out_idx, workspace
+
+
+def knn_scores(queries, base, *, metric: int = RK_KNN_IP, out=None, stream=None):
+    """The [nq, n] float32 scores rk_knn_topk selects from, written to memory
+    (same arithmetic: the unfused path)."""
+    torch = _torch()
+    dt = _dtype_code(queries)
+    nq, d = queries.shape
+    n = base.shape[0]
+    if out is None:
+        out = torch.empty((nq, n), dtype=torch.float32, device=queries.device)
+    rc = lib().rk_knn_scores(dt, metric, _ptr(queries.contiguous()), _ptr(base.contiguous()), nq,
+                             n, d, _ptr(out), _stream_ptr(stream))
+    _check(rc, "rk_knn_scores")
+    return out
 
 
 def probe_read(t, blocks_per_cu: int = 0, stream=None) -> None:

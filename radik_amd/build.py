@@ -25,6 +25,7 @@ SOURCES = [
     "rk_gen.hip",
     "rk_tune.cpp",
     "rk_serve.cpp",
+    "rk_knn.hip",
 ]
 
 COMMON = [

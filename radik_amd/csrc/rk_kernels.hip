@@ -466,14 +466,19 @@ __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
     if (ng > ng_cap) ng = ng_cap;                   // and never beyond the LDS
     if (ng < 1) ng = 1;
     const uint32_t s = (uint32_t)(ng * ge);
+    // run g starts at floor(g * (len - ge) / (ng - 1)) (in double: no 64-bit
+    // integer division per key), clamped so that the run stays in the row
+    const double step = ng > 1 ? (double)(len - ge) / (double)(ng - 1) : 0.0;
+    const uint32_t ge32 = (uint32_t)ge;
     constexpr int UN = 8;  // loads in flight per thread
     for (uint32_t i0 = threadIdx.x; i0 < s; i0 += SEL_NT * UN) {
       uint32_t u[UN];
 #pragma unroll
       for (int q = 0; q < UN; ++q) {
         const uint32_t i = i0 + q * SEL_NT;
-        const int64_t g = i / ge, j = i % ge;
-        const int64_t start = ng > 1 ? (g * (len - ge)) / (ng - 1) : 0;
+        const uint32_t g = i / ge32, j = i - g * ge32;
+        int64_t start = (int64_t)((double)g * step);
+        start = start < len - ge ? start : len - ge;
         u[q] = i < s ? (uint32_t)ldg(rb + start + j) : 0u;
       }
 #pragma unroll
@@ -780,6 +785,17 @@ RK_DEV void order_decode(int mode, uint64_t key, uint32_t& sk, uint32_t& pos) {
   }
 }
 
+// Slot of this lane among the lanes with `take`, from one LDS atomic per
+// wave on *cnt (called by whole waves; the result is defined where take).
+RK_DEV uint32_t wave_slot(uint32_t* cnt, bool take) {
+  const uint64_t m = __ballot(take);
+  if (m == 0) return 0;
+  const int fl = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if (lane_id() == fl) base = atomicAdd(cnt, (uint32_t)__popcll(m));
+  return (uint32_t)__builtin_amdgcn_readlane((int)base, fl) + mbcnt(m);
+}
+
 // Sort sbuf[0, m) (re-keyed) and write the row's output plus its tail fill.
 template <int DT, bool LARGEST>
 RK_DEV void sort_and_write(const KParams& p, const RowDesc& d, uint64_t* sbuf, uint32_t m, int mode,
@@ -927,20 +943,20 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
 #pragma unroll
           for (int j = 0; j < CH; ++j) {
             const uint32_t i = c0 + threadIdx.x + j * SEL_NT;
-            if (i < n && reg[j] >= kth) {
-              const uint32_t slot = atomicAdd(&slot_cnt, 1u);
-              if (slot < k) sbuf[slot] = order_key<DT>(mode, (uint32_t)(reg[j] >> 32), ~(uint32_t)reg[j]);
-            }
+            const bool take = i < n && reg[j] >= kth;
+            const uint32_t slot = wave_slot(&slot_cnt, take);
+            if (take && slot < k)
+              sbuf[slot] = order_key<DT>(mode, (uint32_t)(reg[j] >> 32), ~(uint32_t)reg[j]);
           }
         }
       } else {
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += SEL_NT) {
-          const uint64_t x = ldg(keys + i);
-          if (x >= kth) {
-            const uint32_t slot = atomicAdd(&slot_cnt, 1u);
-            if (slot < k) sbuf[slot] = order_key<DT>(mode, (uint32_t)(x >> 32), ~(uint32_t)x);
-          }
+        for (uint32_t c0 = 0; c0 < n; c0 += SEL_NT) {  // uniform trip count
+          const uint32_t i = c0 + threadIdx.x;
+          const uint64_t x = i < n ? ldg(keys + i) : 0ull;
+          const bool take = i < n && x >= kth;
+          const uint32_t slot = wave_slot(&slot_cnt, take);
+          if (take && slot < k) sbuf[slot] = order_key<DT>(mode, (uint32_t)(x >> 32), ~(uint32_t)x);
         }
       }
       __syncthreads();

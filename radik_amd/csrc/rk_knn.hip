@@ -1,0 +1,544 @@
+// Fused k-nearest-neighbour select on MI355X: the top-k of every query's
+// scores against a database, with the scores computed by MFMA (bf16 or f16
+// inputs, fp32 accumulation) and filtered against a per-query threshold in
+// the same kernel, so that the [queries x database] score matrix never goes
+// to HBM.  The surviving candidates are selected exactly with the rules of
+// the batched top-k (composite key = (selection key << 32) | ~index, so ties
+// go to the lower database index; sorted output best first).
+//
+// Launch sequence of one call (device-side decisions only, no host sync, no
+// memset: graph-capturable):
+//   1. k_knn<SCORES>   scores against S evenly spaced database vectors
+//   2. k_knn_threshold per query, over its S sample keys: T_safe = the k-th
+//                      best (at least k database scores reach it, since the
+//                      sample is a subset) and T = the r-th best with
+//                      r = ceil(mu + 4 sqrt(mu) + 2), mu = S k / n (about r n / S
+//                      candidates instead of k n / S)
+//   3. k_knn<FILTER>   all n scores; keys >= T appended as candidates
+//   4. k_knn_select    exact top-k of the candidates; a query with fewer than
+//                      k candidates (or more than its slots) is queued for a
+//                      retry with T_safe
+//   5. k_knn<FILTER>   the queued queries only (returns at once if none)
+//   6. k_knn_select    the queued queries; still too many candidates (only
+//                      with massive ties at T_safe) sets the status word
+//
+// Scores: inner product q.x (largest first), or squared L2 distance
+// |q|^2 - (2 q.x - |x|^2) (smallest first), in fp32 with a fixed operation
+// order shared by the fused and the materialising (rk_knn_scores) paths.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <mutex>
+
+#include "../../include/radik_amd.h"
+#include "rk_device.hpp"
+#include "rk_pipeline.hpp"
+
+namespace rk {
+namespace {
+
+constexpr int KNN_NT = 256;       // 4 waves; each owns 32 queries
+constexpr int KNN_BQ = 128;       // queries per workgroup
+constexpr int KNN_DMAX = 256;     // query fragments are held in registers
+constexpr int KNN_SAMPLE = 16384; // sampled database vectors (at most)
+constexpr int KNN_SEL_NT = 512;   // threads of the per-query kernels
+constexpr int KNN_K_MAX = 16384;  // k bound (LDS sort of the selection)
+
+enum KnnMode : int { MODE_SCORES = 0, MODE_FILTER = 1 };
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+template <int DT>
+struct Mfma;
+template <>
+struct Mfma<BF16> {
+  typedef __bf16 v8 __attribute__((ext_vector_type(8)));
+  static RK_DEV f32x16 run(const uint4& a, const uint4& b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8, a),
+                                                   __builtin_bit_cast(v8, b), c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<F16> {
+  typedef _Float16 v8 __attribute__((ext_vector_type(8)));
+  static RK_DEV f32x16 run(const uint4& a, const uint4& b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(v8, a),
+                                                  __builtin_bit_cast(v8, b), c, 0, 0, 0);
+  }
+};
+
+// Sum of squares of the 8 elements of a 16-byte fragment, in element order.
+template <int DT>
+RK_DEV float sq8(const uint4& x, float acc) {
+  const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const float f = bits_to_float<DT>((w[i] >> (16 * h)) & 0xffffu);
+      acc = __fadd_rn(acc, __fmul_rn(f, f));
+    }
+  }
+  return acc;
+}
+
+struct KnnArgs {
+  const void* q;         // queries [nq][d]
+  const void* x;         // database vectors, vector j at x + j * ldx elements
+  int32_t nq;            // queries (list entries when `list` is set)
+  int32_t d;
+  int64_t nv;            // database vectors of this launch
+  int64_t ldx;           // elements between consecutive vectors
+  int32_t metric;        // RK_KNN_IP / RK_KNN_L2
+  int32_t chunk_v;       // vectors per workgroup (multiple of 32)
+  int32_t qblocks;       // ceil(nq / KNN_BQ)
+  int32_t chunks;        // ceil(nv / chunk_v)
+  // scores mode
+  float* out;            // [nq][ldo]
+  int64_t ldo;
+  // filter mode
+  const uint32_t* list;  // query of list entry e (null: e itself)
+  const uint32_t* nlist; // device count of list entries (null: nq)
+  const uint32_t* T;     // per-query threshold key
+  uint32_t* cnt;         // per-query candidate counters
+  uint64_t* cand;        // per-query candidate slots (cap each)
+  uint32_t cap;
+};
+
+// Score of one accumulator element: inner product, or the squared L2
+// distance |q|^2 - (2 q.x - |x|^2) (the same operations in every path).
+RK_DEV float knn_value(int metric, float dot, float qn, float xn) {
+  if (metric == RK_KNN_IP) return dot;
+  return __fsub_rn(qn, __fsub_rn(__fmul_rn(2.0f, dot), xn));
+}
+
+// MFMA tiles: 32 queries x 32 vectors per wave and step (v_mfma_f32_32x32x16,
+// K = 16 per instruction).  Lane l holds A row l % 32 and B column l % 32 at
+// k = 16 t + 8 (l / 32) .. +7, and accumulator i of row 8 (i / 4) + 4 (l / 32)
+// + i % 4, column l % 32 (the layout was checked on MI355X).
+template <int DT, int KS, int MODE>
+__global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
+  using U = typename Traits<DT>::U;
+  __shared__ float sh_qn[KNN_NT / 64][32];
+  __shared__ uint32_t sh_t[KNN_NT / 64][32];
+  __shared__ uint32_t sh_q[KNN_NT / 64][32];
+  const int lane = lane_id();
+  const int w = threadIdx.x >> 6;
+  // Workgroup -> (query block, chunk): the query blocks of one chunk run on
+  // one XCD (workgroup i goes to XCD i % 8), so its vectors are read from
+  // HBM once into that XCD's L2 and shared.
+  const int id = blockIdx.x;
+  const int xcd = id & 7;
+  const int local = id >> 3;
+  const int qb = local % a.qblocks;
+  const int chunk = (local / a.qblocks) * 8 + xcd;
+  if (chunk >= a.chunks) return;
+  const int32_t nlist = a.nlist ? (int32_t)*a.nlist : a.nq;
+  const int32_t e0 = qb * KNN_BQ + w * 32;  // first list entry of this wave
+  if (qb * KNN_BQ >= nlist) return;          // uniform over the workgroup
+  const int nsteps = a.d / 16;
+  const U* Q = static_cast<const U*>(a.q);
+  const U* X = static_cast<const U*>(a.x);
+
+  // This lane's A row and its fragments.
+  const int32_t ea = e0 + (lane & 31);
+  const bool va = ea < nlist;
+  const uint32_t qa = va ? (a.list ? a.list[ea] : (uint32_t)ea) : 0u;
+  uint4 af[KS];
+  float qpart = 0.0f;
+#pragma unroll
+  for (int t = 0; t < KS; ++t) {
+    af[t] = make_uint4(0u, 0u, 0u, 0u);
+    if (t < nsteps && va)
+      af[t] = *reinterpret_cast<const uint4*>(Q + (size_t)qa * a.d + t * 16 + (lane >> 5) * 8);
+    if (t < nsteps) qpart = sq8<DT>(af[t], qpart);
+  }
+  const float qn = __fadd_rn(qpart, __shfl_xor(qpart, 32, 64));
+  if (lane < 32) {
+    sh_qn[w][lane] = qn;
+    sh_q[w][lane] = va ? qa : 0xffffffffu;
+    sh_t[w][lane] = (MODE == MODE_FILTER && va) ? a.T[qa] : 0u;
+  }
+  __syncthreads();
+  // accumulator i of this lane belongs to row rowof(i) of the wave's 32
+  auto rowof = [&](int i) { return 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3); };
+  const bool largest = (a.metric == RK_KNN_IP);
+  const int64_t v0 = (int64_t)chunk * a.chunk_v;
+  const int64_t v1 = std::min<int64_t>(a.nv, v0 + a.chunk_v);
+  for (int64_t vt = v0; vt < v1; vt += 32) {
+    const int64_t vb = vt + (lane & 31);
+    const bool vv = vb < v1;
+    uint4 bf[KS];
+    float xpart = 0.0f;
+#pragma unroll
+    for (int t = 0; t < KS; ++t) {
+      bf[t] = make_uint4(0u, 0u, 0u, 0u);
+      if (t < nsteps && vv)
+        bf[t] = *reinterpret_cast<const uint4*>(X + vb * a.ldx + t * 16 + (lane >> 5) * 8);
+    }
+    f32x16 c = {};
+#pragma unroll
+    for (int t = 0; t < KS; ++t)
+      if (t < nsteps) c = Mfma<DT>::run(af[t], bf[t], c);
+    if (a.metric == RK_KNN_L2) {
+#pragma unroll
+      for (int t = 0; t < KS; ++t)
+        if (t < nsteps) xpart = sq8<DT>(bf[t], xpart);
+    }
+    const float xn = __fadd_rn(xpart, __shfl_xor(xpart, 32, 64));
+    auto key_of = [&](int i) -> uint32_t {
+      const uint32_t bits = __float_as_uint(knn_value(a.metric, c[i], sh_qn[w][rowof(i)], xn));
+      return largest ? sel_key<F32, true>(bits) : sel_key<F32, false>(bits);
+    };
+    if constexpr (MODE == MODE_SCORES) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const uint32_t q = sh_q[w][rowof(i)];
+        if (vv && q != 0xffffffffu)
+          a.out[(size_t)q * a.ldo + vb] = knn_value(a.metric, c[i], sh_qn[w][rowof(i)], xn);
+      }
+    } else {
+      uint32_t hit = 0;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const bool ok = vv && sh_q[w][rowof(i)] != 0xffffffffu && key_of(i) >= sh_t[w][rowof(i)];
+        hit |= ok ? (1u << i) : 0u;
+      }
+      if (__ballot(hit != 0)) {  // candidates are rare: one atomic each
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          if ((hit >> i) & 1u) {
+            const uint32_t q = sh_q[w][rowof(i)];
+            const uint32_t slot = atomicAdd(&a.cnt[q], 1u);
+            if (slot < a.cap)
+              a.cand[(size_t)q * a.cap + slot] = ((uint64_t)key_of(i) << 32) | (uint32_t)(~(uint32_t)vb);
+          }
+        }
+      }
+    }
+  }
+}
+
+// Per query: T_safe = the k-th best of the S sample keys, T = the r-th best.
+// Also resets the query's candidate counter and (block 0) the header.
+__global__ __launch_bounds__(KNN_SEL_NT) void k_knn_threshold(const float* sample, int32_t S,
+                                                              int32_t nq, int64_t n, int32_t k,
+                                                              int32_t metric, uint32_t* T,
+                                                              uint32_t* T_safe, uint32_t* cnt,
+                                                              Header* hdr) {
+  __shared__ uint32_t keys[KNN_SAMPLE];
+  __shared__ uint32_t hist[HIST_MAX_BINS];
+  __shared__ uint32_t scratch[16];
+  __shared__ uint64_t scratch64[16];
+  __shared__ uint32_t outv[4];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    hdr->status = 0;
+    hdr->fallback_rows = 0;
+    hdr->ordered_rows = 0;
+    hdr->total_cand = 0;
+  }
+  const bool largest = (metric == RK_KNN_IP);
+  const float mu = (float)S * (float)k / (float)n;
+  const float rf = ceilf(mu + 4.0f * sqrtf(mu) + 2.0f);
+  const uint32_t r = (S == n || rf >= (float)k) ? (uint32_t)k : (uint32_t)rf;
+  for (int q = blockIdx.x; q < nq; q += gridDim.x) {
+    const float* row = sample + (size_t)q * S;
+    for (int i = threadIdx.x; i < S; i += KNN_SEL_NT) {
+      const uint32_t bits = __float_as_uint(row[i]);
+      keys[i] = largest ? sel_key<F32, true>(bits) : sel_key<F32, false>(bits);
+    }
+    __syncthreads();
+    const uint32_t ts =
+        block_radix_select<KNN_SEL_NT, 11, uint32_t, false>(keys, S, k, hist, scratch, scratch64, outv);
+    const uint32_t tp = r < (uint32_t)k
+        ? block_radix_select<KNN_SEL_NT, 11, uint32_t, false>(keys, S, r, hist, scratch, scratch64, outv)
+        : ts;
+    if (threadIdx.x == 0) {
+      T_safe[q] = ts;
+      T[q] = tp;
+      cnt[q] = 0;
+    }
+    __syncthreads();
+  }
+}
+
+// Per query: the exact top-k of its candidates (keys >= T), written best
+// first (sorted) or in database order.  Pass 0 queues a query whose
+// candidates cannot hold its top-k for a retry with T_safe; pass 1 reports it.
+__global__ __launch_bounds__(KNN_SEL_NT) void k_knn_select(int pass, const uint32_t* list,
+                                                           const uint32_t* nlist_dev, int32_t nq,
+                                                           int32_t k, int32_t metric, int sorted,
+                                                           uint32_t* T, const uint32_t* T_safe,
+                                                           uint32_t* cnt, const uint64_t* cand,
+                                                           uint32_t cap, int32_t lds_keys,
+                                                           uint32_t* faillist, Header* hdr,
+                                                           float* out_v, int32_t* out_i) {
+  extern __shared__ uint64_t sbuf[];
+  __shared__ uint32_t hist[HIST_MAX_BINS];
+  __shared__ uint32_t scratch[16];
+  __shared__ uint64_t scratch64[16];
+  __shared__ uint32_t outv[4];
+  __shared__ uint32_t slot_cnt;
+  const int32_t nl = nlist_dev ? (int32_t)*nlist_dev : nq;
+  const bool largest = (metric == RK_KNN_IP);
+  for (int e = blockIdx.x; e < nl; e += gridDim.x) {
+    const uint32_t q = list ? list[e] : (uint32_t)e;
+    const uint32_t n = cnt[q];
+    if (n < (uint32_t)k || n > cap) {
+      if (threadIdx.x == 0) {
+        if (pass == 0) {
+          faillist[atomicAdd(&hdr->fallback_rows, 1u)] = q;
+          T[q] = T_safe[q];
+          cnt[q] = 0;
+        } else {
+          atomicOr(&hdr->status, (uint32_t)ST_CANDIDATES);
+        }
+      }
+      __syncthreads();
+      continue;
+    }
+    const uint64_t* src = cand + (size_t)q * cap;
+    const bool in_lds = n <= (uint32_t)lds_keys;
+    if (in_lds)
+      for (uint32_t i = threadIdx.x; i < n; i += KNN_SEL_NT) sbuf[i] = src[i];
+    __syncthreads();
+    const uint64_t kth =
+        in_lds ? block_radix_select<KNN_SEL_NT, 11, uint64_t, false>(sbuf, n, k, hist, scratch, scratch64, outv)
+               : block_radix_select<KNN_SEL_NT, 11, uint64_t, true>(src, n, k, hist, scratch, scratch64, outv);
+    if (threadIdx.x == 0) slot_cnt = 0;
+    __syncthreads();
+    // Keep the k keys >= kth (composite keys are distinct), re-keyed so that
+    // one descending sort gives the output order: best first, or database
+    // order ((~index << 32) | key).
+    constexpr int CH = 4;
+    for (uint32_t c0 = 0; c0 < n; c0 += KNN_SEL_NT * CH) {
+      uint64_t reg[CH];
+#pragma unroll
+      for (int j = 0; j < CH; ++j) {
+        const uint32_t i = c0 + threadIdx.x + j * KNN_SEL_NT;
+        reg[j] = i < n ? (in_lds ? sbuf[i] : src[i]) : 0ull;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < CH; ++j) {
+        const uint32_t i = c0 + threadIdx.x + j * KNN_SEL_NT;
+        const bool take = i < n && reg[j] >= kth;
+        const uint64_t m = __ballot(take);
+        if (m) {
+          const int fl = __ffsll((unsigned long long)m) - 1;
+          uint32_t base = 0;
+          if (lane_id() == fl) base = atomicAdd(&slot_cnt, (uint32_t)__popcll(m));
+          base = (uint32_t)__builtin_amdgcn_readlane((int)base, fl) + mbcnt(m);
+          if (take && base < (uint32_t)k)
+            sbuf[base] = sorted ? reg[j]
+                                : (((uint64_t)(uint32_t)reg[j] << 32) | (reg[j] >> 32));
+        }
+      }
+    }
+    __syncthreads();
+    const uint32_t P2 = next_pow2((uint32_t)k);
+    for (uint32_t i = (uint32_t)k + threadIdx.x; i < P2; i += KNN_SEL_NT) sbuf[i] = 0;
+    __syncthreads();
+    if (P2 > 1) block_bitonic_desc<KNN_SEL_NT>(sbuf, P2);
+    for (uint32_t i = threadIdx.x; i < (uint32_t)k; i += KNN_SEL_NT) {
+      const uint64_t v = sbuf[i];
+      const uint32_t sk = sorted ? (uint32_t)(v >> 32) : (uint32_t)v;
+      const uint32_t idx = ~(sorted ? (uint32_t)v : (uint32_t)(v >> 32));
+      const uint32_t bits = largest ? key_to_bits<F32, true>(sk) : key_to_bits<F32, false>(sk);
+      out_v[(size_t)q * k + i] = __uint_as_float(bits);
+      out_i[(size_t)q * k + i] = (int32_t)idx;
+    }
+    __syncthreads();
+  }
+}
+
+struct KnnLayout {
+  size_t hdr, T, T_safe, cnt, fail, big, total;
+  int32_t S;
+  uint32_t cap;
+};
+
+size_t a256(size_t x) { return (x + 255) & ~size_t(255); }
+
+uint32_t next_pow2_host(uint32_t x) {
+  uint32_t p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+KnnLayout knn_layout(int32_t nq, int64_t n, int32_t k) {
+  KnnLayout L{};
+  L.S = (int32_t)std::min<int64_t>(n, KNN_SAMPLE);
+  // candidate slots per query: twice the expected count at T_safe, plus
+  // 4096, at most n
+  const double exp_safe = (double)k * (double)n / (double)L.S;
+  L.cap = (uint32_t)std::min<double>((double)n, 2.0 * exp_safe + 4096.0);
+  const size_t Q = (size_t)std::max(nq, 1);
+  size_t o = 0;
+  L.hdr = o; o = a256(o + sizeof(Header));
+  L.T = o; o = a256(o + Q * 4);
+  L.T_safe = o; o = a256(o + Q * 4);
+  L.cnt = o; o = a256(o + Q * 4);
+  L.fail = o; o = a256(o + Q * 4);
+  // the sample scores are dead once the thresholds exist: the candidate
+  // slots reuse their bytes
+  L.big = o;
+  o = a256(o + std::max(Q * (size_t)L.S * 4, Q * (size_t)L.cap * 8));
+  L.total = o;
+  return L;
+}
+
+template <int DT, int KS, int MODE>
+hipError_t launch_knn(const KnnArgs& a0, hipStream_t s) {
+  KnnArgs a = a0;
+  const int64_t tiles = (a.nv + 31) / 32;
+  const int64_t qblocks = (a.nq + KNN_BQ - 1) / KNN_BQ;
+  // about 2048 workgroups: chunk = tiles * qblocks / 2048 tiles, in [8, 256]
+  int64_t ct = (tiles * qblocks + 2047) / 2048;
+  ct = std::max<int64_t>(8, std::min<int64_t>(256, ct));
+  ct = std::min<int64_t>(ct, std::max<int64_t>(tiles, 1));
+  a.chunk_v = (int32_t)(ct * 32);
+  a.qblocks = (int32_t)std::max<int64_t>(qblocks, 1);
+  a.chunks = (int32_t)((a.nv + a.chunk_v - 1) / a.chunk_v);
+  const int64_t per_xcd = (int64_t)a.qblocks * ((a.chunks + 7) / 8);
+  const int64_t grid = per_xcd * 8;
+  if (grid <= 0 || a.nq <= 0) return hipSuccess;
+  k_knn<DT, KS, MODE><<<(unsigned)grid, KNN_NT, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+template <int DT, int MODE>
+hipError_t launch_knn_d(const KnnArgs& a, hipStream_t s) {
+  if (a.d <= 64) return launch_knn<DT, 4, MODE>(a, s);
+  if (a.d <= 128) return launch_knn<DT, 8, MODE>(a, s);
+  return launch_knn<DT, 16, MODE>(a, s);
+}
+
+template <int MODE>
+hipError_t launch_knn_t(int dtype, const KnnArgs& a, hipStream_t s) {
+  return dtype == RK_BF16 ? launch_knn_d<BF16, MODE>(a, s) : launch_knn_d<F16, MODE>(a, s);
+}
+
+int check_common(int dtype, int metric, const void* queries, const void* base, int32_t nq, int64_t n,
+                 int32_t d) {
+  if (dtype != RK_BF16 && dtype != RK_F16) return RK_E_UNSUPPORTED;
+  if (metric != RK_KNN_IP && metric != RK_KNN_L2) return RK_E_ARG;
+  if (nq < 0 || n < 1 || n > INT32_MAX || d < 16 || d > KNN_DMAX || d % 16) return RK_E_ARG;
+  if ((nq && !queries) || !base) return RK_E_ARG;
+  if (((uintptr_t)queries % 16) || ((uintptr_t)base % 16)) return RK_E_ALIGN;
+  return RK_OK;
+}
+
+}  // namespace
+}  // namespace rk
+
+using namespace rk;
+
+extern "C" int rk_knn_workspace_size(int dtype, int32_t nq, int64_t n, int32_t d, int32_t k,
+                                     size_t* bytes) {
+  if (!bytes) return RK_E_ARG;
+  if (dtype != RK_BF16 && dtype != RK_F16) return RK_E_UNSUPPORTED;
+  if (nq < 0 || n < 1 || n > INT32_MAX || d < 16 || d > KNN_DMAX || d % 16) return RK_E_ARG;
+  if (k < 1 || k > KNN_K_MAX || k > n) return RK_E_ARG;
+  *bytes = knn_layout(nq, n, k).total;
+  return RK_OK;
+}
+
+extern "C" int rk_knn_scores(int dtype, int metric, const void* queries, const void* base,
+                             int32_t nq, int64_t n, int32_t d, float* out, void* stream) {
+  const int rc = check_common(dtype, metric, queries, base, nq, n, d);
+  if (rc != RK_OK) return rc;
+  if (nq && !out) return RK_E_ARG;
+  if ((uintptr_t)out % 4) return RK_E_ALIGN;
+  KnnArgs a{};
+  a.q = queries;
+  a.x = base;
+  a.nq = nq;
+  a.d = d;
+  a.nv = n;
+  a.ldx = d;
+  a.metric = metric;
+  a.out = out;
+  a.ldo = n;
+  return launch_knn_t<MODE_SCORES>(dtype, a, static_cast<hipStream_t>(stream)) == hipSuccess
+             ? RK_OK
+             : RK_E_HIP;
+}
+
+extern "C" int rk_knn_topk(int dtype, int metric, const void* queries, const void* base,
+                           int32_t nq, int64_t n, int32_t d, int32_t k, uint32_t flags,
+                           float* out_vals, int32_t* out_idx, void* workspace, size_t ws_bytes,
+                           void* stream) {
+  int rc = check_common(dtype, metric, queries, base, nq, n, d);
+  if (rc != RK_OK) return rc;
+  if (k < 1 || k > KNN_K_MAX || k > n) return RK_E_ARG;
+  if (flags & ~(uint32_t)RK_SORTED) return RK_E_ARG;
+  if (nq && (!out_vals || !out_idx)) return RK_E_ARG;
+  if (((uintptr_t)out_vals % 4) || ((uintptr_t)out_idx % 4)) return RK_E_ALIGN;
+  if (!workspace) return RK_E_WORKSPACE;
+  if ((uintptr_t)workspace % 256) return RK_E_ALIGN;
+  const KnnLayout L = knn_layout(nq, n, k);
+  if (ws_bytes < L.total) return RK_E_WORKSPACE;
+  if (nq == 0) return RK_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  char* ws = static_cast<char*>(workspace);
+  Header* hdr = reinterpret_cast<Header*>(ws + L.hdr);
+  uint32_t* T = reinterpret_cast<uint32_t*>(ws + L.T);
+  uint32_t* T_safe = reinterpret_cast<uint32_t*>(ws + L.T_safe);
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(ws + L.cnt);
+  uint32_t* fail = reinterpret_cast<uint32_t*>(ws + L.fail);
+  float* sample = reinterpret_cast<float*>(ws + L.big);
+  uint64_t* cand = reinterpret_cast<uint64_t*>(ws + L.big);
+  const uint32_t* nfail = &hdr->fallback_rows;
+  const int lds_keys = std::max(4096, (int)next_pow2_host((uint32_t)k));
+  const size_t dyn = (size_t)lds_keys * 8;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_knn_select),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, KNN_K_MAX * 8);
+  });
+  // 1. sample scores: S vectors at stride floor(n / S)
+  KnnArgs a{};
+  a.q = queries;
+  a.d = d;
+  a.nq = nq;
+  a.metric = metric;
+  const int64_t stride = n / L.S;
+  a.x = base;
+  a.nv = L.S;
+  a.ldx = stride * d;
+  a.out = sample;
+  a.ldo = L.S;
+  if (launch_knn_t<MODE_SCORES>(dtype, a, s) != hipSuccess) return RK_E_HIP;
+  // 2. thresholds
+  const int tgrid = std::min(nq, 4096);
+  k_knn_threshold<<<tgrid, KNN_SEL_NT, 0, s>>>(sample, L.S, nq, n, k, metric, T, T_safe, cnt, hdr);
+  if (hipGetLastError() != hipSuccess) return RK_E_HIP;
+  // 3./5. fused filter; 4./6. select
+  const int sgrid = std::min(nq, 8192);
+  for (int pass = 0; pass < 2; ++pass) {
+    KnnArgs f{};
+    f.q = queries;
+    f.x = base;
+    f.nq = nq;
+    f.d = d;
+    f.nv = n;
+    f.ldx = d;
+    f.metric = metric;
+    f.list = pass ? fail : nullptr;
+    f.nlist = pass ? nfail : nullptr;
+    f.T = T;
+    f.cnt = cnt;
+    f.cand = cand;
+    f.cap = L.cap;
+    if (launch_knn_t<MODE_FILTER>(dtype, f, s) != hipSuccess) return RK_E_HIP;
+    k_knn_select<<<sgrid, KNN_SEL_NT, dyn, s>>>(pass, pass ? fail : nullptr, pass ? nfail : nullptr,
+                                                nq, k, metric, (flags & RK_SORTED) ? 1 : 0, T, T_safe,
+                                                cnt, cand, L.cap, lds_keys, fail, hdr, out_vals,
+                                                out_idx);
+    if (hipGetLastError() != hipSuccess) return RK_E_HIP;
+  }
+  return RK_OK;
+}

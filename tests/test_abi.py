@@ -98,6 +98,30 @@ def test_batched_rejects_bad_arguments_before_touching_the_device(L):
     assert L.rk_topk_compat(0, 1, None, 2, 3, 8, 0, None, 1, 4, 0, None) == -1
 
 
+def test_knn_host_side_validation(L):
+    sz = ctypes.c_size_t()
+    assert L.rk_knn_workspace_size(rk.RK_BF16, 1024, 1_000_000, 128, 100, ctypes.byref(sz)) == 0
+    big = sz.value
+    assert L.rk_knn_workspace_size(rk.RK_BF16, 2048, 1_000_000, 128, 100, ctypes.byref(sz)) == 0
+    assert sz.value > big  # grows with the queries
+    assert L.rk_knn_workspace_size(rk.RK_F32, 4, 100, 64, 5, ctypes.byref(sz)) == -3
+    assert L.rk_knn_workspace_size(rk.RK_F16, 4, 100, 40, 5, ctypes.byref(sz)) == -1  # d % 16
+    assert L.rk_knn_workspace_size(rk.RK_F16, 4, 100, 272, 5, ctypes.byref(sz)) == -1  # d > 256
+    assert L.rk_knn_workspace_size(rk.RK_F16, 4, 100, 64, 101, ctypes.byref(sz)) == -1  # k > n
+    assert L.rk_knn_workspace_size(rk.RK_F16, 4, 100, 64, 0, ctypes.byref(sz)) == -1
+    # rk_knn_topk / rk_knn_scores validate before any launch
+    assert L.rk_knn_topk(rk.RK_BF16, 2, 16, 32, 4, 100, 64, 5, 2, 48, 64, 256, 1 << 20,
+                         None) == -1  # unknown metric
+    assert L.rk_knn_topk(rk.RK_BF16, 0, 16, 32, 4, 100, 64, 5, 2, 48, 64, None, 0, None) == -2
+    assert L.rk_knn_topk(rk.RK_BF16, 0, 17, 32, 4, 100, 64, 5, 2, 48, 64, 256, 1 << 20,
+                         None) == -4  # queries not 16-byte aligned
+    assert L.rk_knn_topk(rk.RK_BF16, 0, 16, 32, 4, 100, 64, 5, 2, 48, 64, 256, 16,
+                         None) == -2  # workspace too small
+    assert L.rk_knn_topk(rk.RK_BF16, 0, 16, 32, 4, 100, 64, 5, 1, 48, 64, 256, 1 << 20,
+                         None) == -1  # flags other than RK_SORTED
+    assert L.rk_knn_scores(rk.RK_BF16, 0, 16, 32, 4, 100, 64, None, None) == -1
+
+
 def test_strerror(L):
     for code in (0, -1, -2, -3, -4, -5, -6, 42):
         assert L.rk_strerror(code)
