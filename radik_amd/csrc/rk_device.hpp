@@ -275,6 +275,28 @@ RK_DEV uint64_t block_or64(uint64_t x, uint64_t* scratch) {
   return r;
 }
 
+// dst[i] = f(src[i], i) for i < n, over the workgroup, with UN global loads
+// in flight per thread: all loads of a batch are issued before the first
+// LDS store, so a copy of n elements costs ceil(n / (NT * UN)) memory round
+// trips instead of ceil(n / NT).  (A plain load -> store loop waits for each
+// load before its store.)
+template <int NT, int UN, typename S, typename D, typename F>
+RK_DEV void load_to_lds(D* dst, const S* src, uint32_t n, F&& f) {
+  for (uint32_t i0 = threadIdx.x; i0 < n; i0 += NT * UN) {
+    S x[UN];
+#pragma unroll
+    for (int u = 0; u < UN; ++u) {
+      const uint32_t i = i0 + (uint32_t)u * NT;
+      x[u] = i < n ? ldg(src + i) : S(0);
+    }
+#pragma unroll
+    for (int u = 0; u < UN; ++u) {
+      const uint32_t i = i0 + (uint32_t)u * NT;
+      if (i < n) dst[i] = f(x[u], i);
+    }
+  }
+}
+
 // Finds, scanning from the top bin down, the bin b with
 //   above(b) < k <= above(b) + hist[b]
 // where above(b) = sum of hist over bins > b.  Result in out[0..2] =

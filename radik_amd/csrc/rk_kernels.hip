@@ -470,7 +470,7 @@ __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
     // integer division per key), clamped so that the run stays in the row
     const double step = ng > 1 ? (double)(len - ge) / (double)(ng - 1) : 0.0;
     const uint32_t ge32 = (uint32_t)ge;
-    constexpr int UN = 8;  // loads in flight per thread
+    constexpr int UN = 16;  // loads in flight per thread
     for (uint32_t i0 = threadIdx.x; i0 < s; i0 += SEL_NT * UN) {
       uint32_t u[UN];
 #pragma unroll
@@ -871,8 +871,9 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
       }
     } else if (ph == PH_COPYALL) {
       const uint32_t m = (uint32_t)(len < (int64_t)k ? len : (int64_t)k);
-      for (uint32_t i = threadIdx.x; i < m; i += SEL_NT)
-        sbuf[i] = order_key<DT>(mode, sel_key<DT, LARGEST>((uint32_t)ldg(rb + i)), i);
+      load_to_lds<SEL_NT, 4>(sbuf, rb, m, [&](U x, uint32_t i) {
+        return order_key<DT>(mode, sel_key<DT, LARGEST>((uint32_t)x), i);
+      });
       __syncthreads();
       sort_and_write<DT, LARGEST>(p, d, sbuf, m, mode);
     } else {
@@ -881,8 +882,9 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
       bool in_lds;
       if (ph == PH_SMALL) {
         n = (uint32_t)len;
-        for (uint32_t i = threadIdx.x; i < n; i += SEL_NT)
-          sbuf[i] = ((uint64_t)sel_key<DT, LARGEST>((uint32_t)ldg(rb + i)) << 32) | (uint32_t)(~i);
+        load_to_lds<SEL_NT, 4>(sbuf, rb, n, [](U x, uint32_t i) {
+          return ((uint64_t)sel_key<DT, LARGEST>((uint32_t)x) << 32) | (uint32_t)(~i);
+        });
         keys = sbuf;
         in_lds = true;
       } else {
@@ -893,17 +895,29 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
         uint32_t why = n > st.cap ? FB_OVERFLOW : FB_TOO_FEW;
         if (ok) {
           in_lds = n <= (uint32_t)p.lds_keys;
+          uint32_t ge = 0;
           if (in_lds) {
-            for (uint32_t i = threadIdx.x; i < n; i += SEL_NT) sbuf[i] = ldg(src + i);
+            load_to_lds<SEL_NT, 4>(sbuf, src, n, [](uint64_t x, uint32_t) { return x; });
             keys = sbuf;
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < n; i += SEL_NT)
+              ge += ((uint32_t)(ld_lds(keys + i) >> 32) >= st.T) ? 1u : 0u;
           } else {
             keys = src;
-          }
-          __syncthreads();
-          uint32_t ge = 0;
-          for (uint32_t i = threadIdx.x; i < n; i += SEL_NT) {
-            const uint64_t x = in_lds ? ld_lds(keys + i) : ldg(keys + i);
-            ge += ((uint32_t)(x >> 32) >= st.T) ? 1u : 0u;
+            constexpr int UN = 4;  // loads in flight per thread
+            for (uint32_t i0 = threadIdx.x; i0 < n; i0 += SEL_NT * UN) {
+              uint64_t x[UN];
+#pragma unroll
+              for (int u = 0; u < UN; ++u) {
+                const uint32_t i = i0 + (uint32_t)u * SEL_NT;
+                x[u] = i < n ? ldg(keys + i) : 0ull;
+              }
+#pragma unroll
+              for (int u = 0; u < UN; ++u) {
+                const uint32_t i = i0 + (uint32_t)u * SEL_NT;
+                ge += (i < n && (uint32_t)(x[u] >> 32) >= st.T) ? 1u : 0u;
+              }
+            }
           }
           ge = block_sum<SEL_NT>(ge, scratch);
           ok = ge >= k;
@@ -951,12 +965,22 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
         }
       } else {
         __syncthreads();
-        for (uint32_t c0 = 0; c0 < n; c0 += SEL_NT) {  // uniform trip count
-          const uint32_t i = c0 + threadIdx.x;
-          const uint64_t x = i < n ? ldg(keys + i) : 0ull;
-          const bool take = i < n && x >= kth;
-          const uint32_t slot = wave_slot(&slot_cnt, take);
-          if (take && slot < k) sbuf[slot] = order_key<DT>(mode, (uint32_t)(x >> 32), ~(uint32_t)x);
+        constexpr int UN = 4;  // loads in flight per thread
+        for (uint32_t c0 = 0; c0 < n; c0 += SEL_NT * UN) {  // uniform trip count
+          uint64_t x[UN];
+#pragma unroll
+          for (int u = 0; u < UN; ++u) {
+            const uint32_t i = c0 + threadIdx.x + (uint32_t)u * SEL_NT;
+            x[u] = i < n ? ldg(keys + i) : 0ull;
+          }
+#pragma unroll
+          for (int u = 0; u < UN; ++u) {
+            const uint32_t i = c0 + threadIdx.x + (uint32_t)u * SEL_NT;
+            const bool take = i < n && x[u] >= kth;
+            const uint32_t slot = wave_slot(&slot_cnt, take);
+            if (take && slot < k)
+              sbuf[slot] = order_key<DT>(mode, (uint32_t)(x[u] >> 32), ~(uint32_t)x[u]);
+          }
         }
       }
       __syncthreads();

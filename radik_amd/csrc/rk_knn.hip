@@ -301,8 +301,7 @@ __global__ __launch_bounds__(KNN_SEL_NT) void k_knn_select(int pass, const uint3
     }
     const uint64_t* src = cand + (size_t)q * cap;
     const bool in_lds = n <= (uint32_t)lds_keys;
-    if (in_lds)
-      for (uint32_t i = threadIdx.x; i < n; i += KNN_SEL_NT) sbuf[i] = src[i];
+    if (in_lds) load_to_lds<KNN_SEL_NT, 8>(sbuf, src, n, [](uint64_t x, uint32_t) { return x; });
     __syncthreads();
     const uint64_t kth =
         in_lds ? block_radix_select<KNN_SEL_NT, 11, uint64_t, false>(sbuf, n, k, hist, scratch, scratch64, outv)
