@@ -71,7 +71,12 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c4_f32", choices=["c4_f32", "c4_bf16", "c2", "c1", "c3"])
+    ap.add_argument("--workload", default="c4_f32",
+                    choices=["c4_f32", "c4_bf16", "c2", "c1", "c3", "knn"],
+                    help="knn: the fused k-nearest-neighbour select against its unfused "
+                         "form (prints its own JSON object)")
+    ap.add_argument("--knn-shape", default="1024,1048576,128,100",
+                    help="knn workload: queries,database vectors,dim,k")
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -103,6 +108,9 @@ def main() -> int:
     torch.cuda.set_device(0)
     if args.serve_only:
         print(json.dumps({"serving": serving(args)}), flush=True)
+        return 0
+    if args.workload == "knn":
+        print(json.dumps({"knn": knn_bench(args)}), flush=True)
         return 0
     w = make_workload(args.workload)
     es = 4 if w.dtype == "f32" else 2
@@ -326,6 +334,94 @@ def main() -> int:
 
     print(json.dumps(result), flush=True)
     return 0
+
+
+def knn_bench(args) -> dict:
+    """Fused k-nearest-neighbour select (rk_knn_topk) on a resident bf16
+    database against its unfused form: the [queries x database] fp32 scores
+    written by rk_knn_scores, then rk_topk_batched over them.  Both give the
+    same bits (checked here on the last call).  MFMA roofline: 2 * nq * n * d
+    flops per call against the bf16 dense peak."""
+    import torch
+
+    import radik_amd as rk
+
+    nq, n, d, k = (int(v) for v in args.knn_shape.split(","))
+    g = torch.Generator(device="cuda").manual_seed(6)
+    q = torch.randn(nq, d, device="cuda", generator=g).to(torch.bfloat16)
+    x = torch.randn(n, d, device="cuda", generator=g).to(torch.bfloat16)
+    flops = 2.0 * nq * n * d
+    peak = 2500.0  # TFLOP/s, MI355X dense bf16
+    steps, warm = max(args.steps, 1), max(args.warmup, 1)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        for _ in range(warm):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    out = {"workload": f"{nq} queries x {n} database vectors, d = {d}, bf16, k = {k}, sorted",
+           "data": "synthetic: N(0,1) rounded to bf16 (torch generator seed 6)",
+           "flops_per_call": flops, "database_bytes": n * d * 2, "steps": steps}
+    scores = torch.empty((nq, n), dtype=torch.float32, device="cuda")
+    off = torch.arange(nq + 1, dtype=torch.int64, device="cuda") * n
+    ks = torch.full((nq,), k, dtype=torch.int32, device="cuda")
+    for name, metric in (("inner_product", rk.RK_KNN_IP), ("l2", rk.RK_KNN_L2)):
+        ws = torch.empty(rk.knn_workspace_size(rk.RK_BF16, nq, n, d, k), dtype=torch.uint8,
+                         device="cuda")
+        fv = torch.empty((nq, k), dtype=torch.float32, device="cuda")
+        fi = torch.empty((nq, k), dtype=torch.int32, device="cuda")
+        largest = metric == rk.RK_KNN_IP
+        flags = rk.RK_SORTED | (rk.RK_LARGEST if largest else 0)
+        tws = torch.empty(rk.workspace_size(rk.RK_F32, nq, nq * n, k, flags), dtype=torch.uint8,
+                          device="cuda")
+        uv = torch.empty(nq * k, dtype=torch.float32, device="cuda")
+        ui = torch.empty(nq * k, dtype=torch.int32, device="cuda")
+
+        def fused():
+            rk.knn_topk(q, x, k, metric=metric, sorted=True, workspace=ws, out_vals=fv,
+                        out_idx=fi)
+
+        def score_only():
+            rk.knn_scores(q, x, metric=metric, out=scores)
+
+        def unfused():
+            score_only()
+            rk.topk_batched(scores.view(-1), off, ks, largest=largest, sorted=True,
+                            total_len=nq * n, max_k_=k, workspace=tws, out_vals=uv, out_idx=ui)
+
+        f_ms = timed(fused)
+        st = rk.status(ws)
+        if st[0] != 0:
+            raise RuntimeError(f"knn status {st}")
+        s_ms = timed(score_only)
+        u_ms = timed(unfused)
+        same = bool(torch.equal(fv.view(-1).view(torch.int32), uv.view(torch.int32))
+                    and torch.equal(fi.view(-1), ui))
+        if not same:
+            raise RuntimeError(f"knn {name}: fused and unfused results differ")
+        out[name] = {
+            "fused_ms": f_ms, "queries_per_s": nq / (f_ms / 1e3),
+            "fused_tflops": flops / (f_ms / 1e3) / 1e12,
+            "fused_frac_of_mfma_peak": flops / (f_ms / 1e3) / 1e12 / peak,
+            "scores_only_ms": s_ms, "scores_tflops": flops / (s_ms / 1e3) / 1e12,
+            "unfused_ms": u_ms, "fused_speedup": u_ms / f_ms,
+            "second_pass_queries": st[1], "bit_exact_vs_unfused": same}
+        log(f"knn {name}: fused {f_ms:.3f} ms ({out[name]['fused_tflops']:.0f} TFLOP/s), "
+            f"scores only {s_ms:.3f} ms, unfused {u_ms:.3f} ms ({u_ms / f_ms:.2f}x)")
+        del ws, tws
+    out["roofline"] = {"bound": "mfma", "peak": peak, "unit": "TFLOP/s",
+                       "achieved": out["inner_product"]["fused_tflops"],
+                       "frac": out["inner_product"]["fused_frac_of_mfma_peak"]}
+    return out
 
 
 def percentile(x, q):

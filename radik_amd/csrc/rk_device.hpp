@@ -149,6 +149,14 @@ RK_DEV uint4 bload16_nt(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
   const rk_v4u v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)byte_off, 0, 2);
   return make_uint4(v.x, v.y, v.z, v.w);
 }
+// 16-byte buffer load, cached normally (data other workgroups read again).
+RK_DEV uint4 bload16(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+  const rk_v4u v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)byte_off, 0, 0);
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+// A byte offset beyond every range a descriptor here describes (loads at it
+// return zeros).
+constexpr uint32_t RSRC_OOB = 0xfffffff0u;
 
 // Best value of a 16-byte vector in the selection direction (max for
 // largest-k, min for smallest-k), as fp32.  max/min ignore a NaN operand, so

@@ -39,11 +39,11 @@
 namespace rk {
 namespace {
 
-constexpr int KNN_NT = 256;       // 4 waves; each owns 32 queries
-constexpr int KNN_BQ = 128;       // queries per workgroup
+constexpr int KNN_NT = 256;       // 4 waves; each owns 32 or 64 queries
 constexpr int KNN_DMAX = 256;     // query fragments are held in registers
 constexpr int KNN_SAMPLE = 16384; // sampled database vectors (at most)
 constexpr int KNN_SEL_NT = 512;   // threads of the per-query kernels
+constexpr int KNN_STAGE = 256;    // staged candidates per wave (filter)
 constexpr int KNN_K_MAX = 16384;  // k bound (LDS sort of the selection)
 
 enum KnnMode : int { MODE_SCORES = 0, MODE_FILTER = 1 };
@@ -93,7 +93,7 @@ struct KnnArgs {
   int64_t ldx;           // elements between consecutive vectors
   int32_t metric;        // RK_KNN_IP / RK_KNN_L2
   int32_t chunk_v;       // vectors per workgroup (multiple of 32)
-  int32_t qblocks;       // ceil(nq / KNN_BQ)
+  int32_t qblocks;       // ceil(nq / queries per workgroup)
   int32_t chunks;        // ceil(nv / chunk_v)
   // scores mode
   float* out;            // [nq][ldo]
@@ -114,16 +114,32 @@ RK_DEV float knn_value(int metric, float dot, float qn, float xn) {
   return __fsub_rn(qn, __fsub_rn(__fmul_rn(2.0f, dot), xn));
 }
 
-// MFMA tiles: 32 queries x 32 vectors per wave and step (v_mfma_f32_32x32x16,
-// K = 16 per instruction).  Lane l holds A row l % 32 and B column l % 32 at
-// k = 16 t + 8 (l / 32) .. +7, and accumulator i of row 8 (i / 4) + 4 (l / 32)
-// + i % 4, column l % 32 (the layout was checked on MI355X).
-template <int DT, int KS, int MODE>
+// MFMA tiles: each wave holds MT tiles of 32 queries and multiplies them by
+// 32 database vectors per step (v_mfma_f32_32x32x16, K = 16 per
+// instruction), so one B fragment feeds MT instructions.  Lane l holds A row
+// l % 32 and B column l % 32 at k = 16 t + 8 (l / 32) .. +7, and accumulator
+// i of row 8 (i / 4) + 4 (l / 32) + i % 4, column l % 32 (the layout was
+// checked on MI355X).
+//
+// Filter mode tests every score with one float compare against its query's
+// threshold, held in registers: !(v < tf) for largest-first (!(v > tf) for
+// smallest-first) is true for every score whose key reaches T (tf is the
+// float of T, NaN when T = 0, so that everything passes), for NaN scores, and
+// for +-0 next to a zero threshold.  Only when some lane of the wave passes
+// (a wave-uniform branch) is the exact key compared and the candidate
+// appended; with thresholds near the k-th best score that is a small
+// fraction of the compares.
+template <int DT, int KS, int MT, int MODE, int METRIC>
 __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
   using U = typename Traits<DT>::U;
-  __shared__ float sh_qn[KNN_NT / 64][32];
-  __shared__ uint32_t sh_t[KNN_NT / 64][32];
-  __shared__ uint32_t sh_q[KNN_NT / 64][32];
+  constexpr int NW = KNN_NT / 64;
+  constexpr int QW = 32 * MT;  // queries per wave
+  constexpr bool LARGEST = (METRIC == RK_KNN_IP);
+  __shared__ float sh_qn[NW][QW];
+  __shared__ uint32_t sh_t[NW][QW];
+  __shared__ uint32_t sh_q[NW][QW];
+  __shared__ uint64_t st_key[MODE == MODE_FILTER ? NW : 1][KNN_STAGE];
+  __shared__ uint32_t st_q[MODE == MODE_FILTER ? NW : 1][KNN_STAGE];
   const int lane = lane_id();
   const int w = threadIdx.x >> 6;
   // Workgroup -> (query block, chunk): the query blocks of one chunk run on
@@ -136,89 +152,144 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
   const int chunk = (local / a.qblocks) * 8 + xcd;
   if (chunk >= a.chunks) return;
   const int32_t nlist = a.nlist ? (int32_t)*a.nlist : a.nq;
-  const int32_t e0 = qb * KNN_BQ + w * 32;  // first list entry of this wave
-  if (qb * KNN_BQ >= nlist) return;          // uniform over the workgroup
+  if (qb * QW * NW >= nlist) return;  // uniform over the workgroup
+  const int32_t e0 = (qb * NW + w) * QW;  // first list entry of this wave
   const int nsteps = a.d / 16;
   const U* Q = static_cast<const U*>(a.q);
   const U* X = static_cast<const U*>(a.x);
 
-  // This lane's A row and its fragments.
-  const int32_t ea = e0 + (lane & 31);
-  const bool va = ea < nlist;
-  const uint32_t qa = va ? (a.list ? a.list[ea] : (uint32_t)ea) : 0u;
-  uint4 af[KS];
-  float qpart = 0.0f;
+  // This lane's A rows and their fragments.
+  uint4 af[MT][KS];
 #pragma unroll
-  for (int t = 0; t < KS; ++t) {
-    af[t] = make_uint4(0u, 0u, 0u, 0u);
-    if (t < nsteps && va)
-      af[t] = *reinterpret_cast<const uint4*>(Q + (size_t)qa * a.d + t * 16 + (lane >> 5) * 8);
-    if (t < nsteps) qpart = sq8<DT>(af[t], qpart);
-  }
-  const float qn = __fadd_rn(qpart, __shfl_xor(qpart, 32, 64));
-  if (lane < 32) {
-    sh_qn[w][lane] = qn;
-    sh_q[w][lane] = va ? qa : 0xffffffffu;
-    sh_t[w][lane] = (MODE == MODE_FILTER && va) ? a.T[qa] : 0u;
-  }
-  __syncthreads();
-  // accumulator i of this lane belongs to row rowof(i) of the wave's 32
-  auto rowof = [&](int i) { return 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3); };
-  const bool largest = (a.metric == RK_KNN_IP);
-  const int64_t v0 = (int64_t)chunk * a.chunk_v;
-  const int64_t v1 = std::min<int64_t>(a.nv, v0 + a.chunk_v);
-  for (int64_t vt = v0; vt < v1; vt += 32) {
-    const int64_t vb = vt + (lane & 31);
-    const bool vv = vb < v1;
-    uint4 bf[KS];
-    float xpart = 0.0f;
+  for (int m = 0; m < MT; ++m) {
+    const int32_t ea = e0 + m * 32 + (lane & 31);
+    const bool va = ea < nlist;
+    const uint32_t qa = va ? (a.list ? a.list[ea] : (uint32_t)ea) : 0u;
+    float qpart = 0.0f;
 #pragma unroll
     for (int t = 0; t < KS; ++t) {
-      bf[t] = make_uint4(0u, 0u, 0u, 0u);
-      if (t < nsteps && vv)
-        bf[t] = *reinterpret_cast<const uint4*>(X + vb * a.ldx + t * 16 + (lane >> 5) * 8);
+      af[m][t] = make_uint4(0u, 0u, 0u, 0u);
+      if (t < nsteps && va)
+        af[m][t] = ldg16<false>(Q + (size_t)qa * a.d + t * 16 + (lane >> 5) * 8);
+      qpart = sq8<DT>(af[m][t], qpart);
     }
-    f32x16 c = {};
+    const float qn = __fadd_rn(qpart, __shfl_xor(qpart, 32, 64));
+    if (lane < 32) {
+      sh_qn[w][m * 32 + lane] = qn;
+      sh_q[w][m * 32 + lane] = va ? qa : 0xffffffffu;
+      sh_t[w][m * 32 + lane] = (MODE == MODE_FILTER && va) ? a.T[qa] : 0u;
+    }
+  }
+  __syncthreads();
+  // accumulator i of this lane belongs to row rowof(i) of a 32-query tile
+  auto rowof = [&](int i) { return 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3); };
+  // per-accumulator float thresholds (filter mode)
+  float tf[MT][16];
+  if constexpr (MODE == MODE_FILTER) {
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int row = m * 32 + rowof(i);
+        const uint32_t T = sh_t[w][row];
+        const float inf = __builtin_inff();
+        tf[m][i] = sh_q[w][row] == 0xffffffffu ? (LARGEST ? inf : -inf)
+                 : T == 0u ? __builtin_nanf("")
+                 : __uint_as_float(key_to_bits<F32, LARGEST>(T));
+      }
+  }
+  const int64_t v0 = (int64_t)chunk * a.chunk_v;
+  const int64_t v1 = std::min<int64_t>(a.nv, v0 + a.chunk_v);
+  // B fragments of the 32 vectors starting at vt, by buffer loads from this
+  // chunk's range: vectors past v1 and steps past d read zeros without a
+  // branch, so the loads of a step are straight-line code and the next
+  // step's can stay in flight (the host keeps a chunk's bytes below 2^32).
+  const __amdgpu_buffer_rsrc_t rs =
+      make_rsrc(X + v0 * a.ldx, v1 > v0 ? (uint32_t)(((v1 - v0 - 1) * a.ldx + a.d) * sizeof(U)) : 0u);
+  auto load_b = [&](uint4 (&bf)[KS], int64_t vt) {
+    const int64_t vb = vt + (lane & 31);
+    const bool vv = vb < v1;
+    const uint32_t base = (uint32_t)((vb - v0) * a.ldx * (int64_t)sizeof(U)) + (uint32_t)(lane >> 5) * 16u;
 #pragma unroll
     for (int t = 0; t < KS; ++t)
-      if (t < nsteps) c = Mfma<DT>::run(af[t], bf[t], c);
-    if (a.metric == RK_KNN_L2) {
-#pragma unroll
-      for (int t = 0; t < KS; ++t)
-        if (t < nsteps) xpart = sq8<DT>(bf[t], xpart);
+      bf[t] = bload16(rs, (t < nsteps && vv) ? base + (uint32_t)t * 32u : RSRC_OOB);
+  };
+  // Candidates of this wave wait in an LDS stage (written and read by this
+  // wave only; LDS operations of one wave complete in order) and go to
+  // their queries' slots in bulk, so the score loop issues no global atomic.
+  uint32_t sn = 0;  // staged entries (wave-uniform)
+  auto flush = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (uint32_t j = (uint32_t)lane; j < sn; j += 64) {
+      const uint32_t q = st_q[w][j];
+      const uint32_t slot = atomicAdd(&a.cnt[q], 1u);
+      if (slot < a.cap) a.cand[(size_t)q * a.cap + slot] = st_key[w][j];
     }
-    const float xn = __fadd_rn(xpart, __shfl_xor(xpart, 32, 64));
-    auto key_of = [&](int i) -> uint32_t {
-      const uint32_t bits = __float_as_uint(knn_value(a.metric, c[i], sh_qn[w][rowof(i)], xn));
-      return largest ? sel_key<F32, true>(bits) : sel_key<F32, false>(bits);
-    };
-    if constexpr (MODE == MODE_SCORES) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    sn = 0;
+  };
+  auto step = [&](const uint4 (&bf)[KS], int64_t vt) {
+    const int64_t vb = vt + (lane & 31);
+    const bool vv = vb < v1;
+    f32x16 c[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      c[m] = f32x16{};
+#pragma unroll
+      for (int t = 0; t < KS; ++t) c[m] = Mfma<DT>::run(af[m][t], bf[t], c[m]);
+    }
+    float xn = 0.0f;
+    if constexpr (METRIC == RK_KNN_L2) {
+      float xpart = 0.0f;
+#pragma unroll
+      for (int t = 0; t < KS; ++t) xpart = sq8<DT>(bf[t], xpart);
+      xn = __fadd_rn(xpart, __shfl_xor(xpart, 32, 64));
+    }
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
-        const uint32_t q = sh_q[w][rowof(i)];
-        if (vv && q != 0xffffffffu)
-          a.out[(size_t)q * a.ldo + vb] = knn_value(a.metric, c[i], sh_qn[w][rowof(i)], xn);
-      }
-    } else {
-      uint32_t hit = 0;
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const bool ok = vv && sh_q[w][rowof(i)] != 0xffffffffu && key_of(i) >= sh_t[w][rowof(i)];
-        hit |= ok ? (1u << i) : 0u;
-      }
-      if (__ballot(hit != 0)) {  // candidates are rare: one atomic each
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          if ((hit >> i) & 1u) {
-            const uint32_t q = sh_q[w][rowof(i)];
-            const uint32_t slot = atomicAdd(&a.cnt[q], 1u);
-            if (slot < a.cap)
-              a.cand[(size_t)q * a.cap + slot] = ((uint64_t)key_of(i) << 32) | (uint32_t)(~(uint32_t)vb);
+        const int row = m * 32 + rowof(i);
+        const float v = (METRIC == RK_KNN_IP) ? c[m][i] : knn_value(METRIC, c[m][i], sh_qn[w][row], xn);
+        if constexpr (MODE == MODE_SCORES) {
+          const uint32_t q = sh_q[w][row];
+          if (vv && q != 0xffffffffu) a.out[(size_t)q * a.ldo + vb] = v;
+        } else {
+          const bool pass = LARGEST ? !(v < tf[m][i]) : !(v > tf[m][i]);
+          if (__ballot(pass)) {  // wave-uniform: some lane may hold a candidate
+            const uint32_t q = sh_q[w][row];
+            const uint32_t key = sel_key<F32, LARGEST>(__float_as_uint(v));
+            const bool take = pass && vv && q != 0xffffffffu && key >= sh_t[w][row];
+            const uint64_t tm = __ballot(take);
+            if (tm) {
+              const uint32_t nt = (uint32_t)__popcll(tm);
+              if (sn + nt > KNN_STAGE) flush();
+              if (take) {
+                const uint32_t pos = sn + mbcnt(tm);
+                st_key[w][pos] = ((uint64_t)key << 32) | (uint32_t)(~(uint32_t)vb);
+                st_q[w][pos] = q;
+              }
+              sn += nt;
+            }
           }
         }
       }
     }
+  };
+  // Two register buffers: the next step's loads are in flight while the
+  // current step's MFMAs and compares run.
+  // The prefetches are unconditional (past v1 they read zeros, and a step
+  // there appends nothing), so no branch splits a load from its use.
+  uint4 b0[KS], b1[KS];
+  load_b(b0, v0);
+  for (int64_t vt = v0; vt < v1; vt += 64) {
+    load_b(b1, vt + 32);
+    step(b0, vt);
+    load_b(b0, vt + 64);
+    step(b1, vt + 32);
   }
+  if constexpr (MODE == MODE_FILTER)
+    if (sn) flush();
 }
 
 // Per query: T_safe = the k-th best of the S sample keys, T = the r-th best.
@@ -389,35 +460,53 @@ KnnLayout knn_layout(int32_t nq, int64_t n, int32_t k) {
   return L;
 }
 
-template <int DT, int KS, int MODE>
+// Tiles per wave: two 32-query tiles share each B fragment when their A
+// fragments fit the register budget (d <= 128), one otherwise.
+template <int KS>
+constexpr int knn_mt() { return KS <= 8 ? 2 : 1; }
+
+template <int DT, int KS, int MODE, int METRIC>
 hipError_t launch_knn(const KnnArgs& a0, hipStream_t s) {
+  constexpr int MT = knn_mt<KS>();
+  constexpr int BQ = 32 * MT * (KNN_NT / 64);  // queries per workgroup
   KnnArgs a = a0;
   const int64_t tiles = (a.nv + 31) / 32;
-  const int64_t qblocks = (a.nq + KNN_BQ - 1) / KNN_BQ;
+  const int64_t qblocks = (a.nq + BQ - 1) / BQ;
   // about 2048 workgroups: chunk = tiles * qblocks / 2048 tiles, in [8, 256]
   int64_t ct = (tiles * qblocks + 2047) / 2048;
   ct = std::max<int64_t>(8, std::min<int64_t>(256, ct));
   ct = std::min<int64_t>(ct, std::max<int64_t>(tiles, 1));
+  // a chunk's byte range (one buffer descriptor) stays below 2^32; 32
+  // vectors always fit, since a vector stride is at most 2^31 / 2^14
+  // elements of 2 bytes
+  ct = std::min<int64_t>(ct, std::max<int64_t>(1, (int64_t)0xffffff00 / (a.ldx * 2) / 32));
   a.chunk_v = (int32_t)(ct * 32);
   a.qblocks = (int32_t)std::max<int64_t>(qblocks, 1);
   a.chunks = (int32_t)((a.nv + a.chunk_v - 1) / a.chunk_v);
   const int64_t per_xcd = (int64_t)a.qblocks * ((a.chunks + 7) / 8);
   const int64_t grid = per_xcd * 8;
   if (grid <= 0 || a.nq <= 0) return hipSuccess;
-  k_knn<DT, KS, MODE><<<(unsigned)grid, KNN_NT, 0, s>>>(a);
+  k_knn<DT, KS, MT, MODE, METRIC><<<(unsigned)grid, KNN_NT, 0, s>>>(a);
   return hipGetLastError();
 }
 
-template <int DT, int MODE>
+template <int DT, int MODE, int METRIC>
 hipError_t launch_knn_d(const KnnArgs& a, hipStream_t s) {
-  if (a.d <= 64) return launch_knn<DT, 4, MODE>(a, s);
-  if (a.d <= 128) return launch_knn<DT, 8, MODE>(a, s);
-  return launch_knn<DT, 16, MODE>(a, s);
+  if (a.d <= 32) return launch_knn<DT, 2, MODE, METRIC>(a, s);
+  if (a.d <= 64) return launch_knn<DT, 4, MODE, METRIC>(a, s);
+  if (a.d <= 128) return launch_knn<DT, 8, MODE, METRIC>(a, s);
+  return launch_knn<DT, 16, MODE, METRIC>(a, s);
+}
+
+template <int DT, int MODE>
+hipError_t launch_knn_m(const KnnArgs& a, hipStream_t s) {
+  return a.metric == RK_KNN_IP ? launch_knn_d<DT, MODE, RK_KNN_IP>(a, s)
+                               : launch_knn_d<DT, MODE, RK_KNN_L2>(a, s);
 }
 
 template <int MODE>
 hipError_t launch_knn_t(int dtype, const KnnArgs& a, hipStream_t s) {
-  return dtype == RK_BF16 ? launch_knn_d<BF16, MODE>(a, s) : launch_knn_d<F16, MODE>(a, s);
+  return dtype == RK_BF16 ? launch_knn_m<BF16, MODE>(a, s) : launch_knn_m<F16, MODE>(a, s);
 }
 
 int check_common(int dtype, int metric, const void* queries, const void* base, int32_t nq, int64_t n,
