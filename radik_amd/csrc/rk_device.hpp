@@ -154,6 +154,10 @@ RK_DEV uint4 bload16(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
   const rk_v4u v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)byte_off, 0, 0);
   return make_uint4(v.x, v.y, v.z, v.w);
 }
+// 4-byte buffer load, cached normally.
+RK_DEV float bload4f(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)byte_off, 0, 0));
+}
 // A byte offset beyond every range a descriptor here describes (loads at it
 // return zeros).
 constexpr uint32_t RSRC_OOB = 0xfffffff0u;

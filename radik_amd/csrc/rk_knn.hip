@@ -14,6 +14,7 @@
 //                      sample is a subset) and T = the r-th best with
 //                      r = ceil(mu + 4 sqrt(mu) + 2), mu = S k / n (about r n / S
 //                      candidates instead of k n / S)
+//   (L2 only: k_knn_xnorm, the squared norm of every database vector)
 //   3. k_knn<FILTER>   all n scores; keys >= T appended as candidates
 //   4. k_knn_select    exact top-k of the candidates; a query with fewer than
 //                      k candidates (or more than its slots) is queued for a
@@ -105,13 +106,17 @@ struct KnnArgs {
   uint32_t* cnt;         // per-query candidate counters
   uint64_t* cand;        // per-query candidate slots (cap each)
   uint32_t cap;
+  const float* xnorm;    // squared norms of the launch's vectors (L2 filter
+                         // mode; scores mode computes them from the B fragments)
 };
 
 // Score of one accumulator element: inner product, or the squared L2
 // distance |q|^2 - (2 q.x - |x|^2) (the same operations in every path).
+// 2 q.x - |x|^2 is one fused multiply-add: 2 q.x is exact in fp32 (short of
+// overflow), so it equals the product-then-difference rounded once.
 RK_DEV float knn_value(int metric, float dot, float qn, float xn) {
   if (metric == RK_KNN_IP) return dot;
-  return __fsub_rn(qn, __fsub_rn(__fmul_rn(2.0f, dot), xn));
+  return __fsub_rn(qn, __fmaf_rn(2.0f, dot, -xn));
 }
 
 // MFMA tiles: each wave holds MT tiles of 32 queries and multiplies them by
@@ -183,6 +188,14 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
   __syncthreads();
   // accumulator i of this lane belongs to row rowof(i) of a 32-query tile
   auto rowof = [&](int i) { return 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3); };
+  // per-accumulator query norms (L2)
+  float qr[MT][16];
+  if constexpr (METRIC == RK_KNN_L2) {
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) qr[m][i] = sh_qn[w][m * 32 + rowof(i)];
+  }
   // per-accumulator float thresholds (filter mode)
   float tf[MT][16];
   if constexpr (MODE == MODE_FILTER) {
@@ -200,19 +213,56 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
   }
   const int64_t v0 = (int64_t)chunk * a.chunk_v;
   const int64_t v1 = std::min<int64_t>(a.nv, v0 + a.chunk_v);
-  // B fragments of the 32 vectors starting at vt, by buffer loads from this
-  // chunk's range: vectors past v1 and steps past d read zeros without a
-  // branch, so the loads of a step are straight-line code and the next
-  // step's can stay in flight (the host keeps a chunk's bytes below 2^32).
+  // B tiles (32 vectors x d) go through LDS once per workgroup: the 256
+  // threads load a tile's 16-byte chunks with buffer loads from this chunk's
+  // range (vectors past v1 and chunks past d read zeros without a branch;
+  // the host keeps a chunk's bytes below 2^32), store them to one of two LDS
+  // buffers, and every wave reads its B fragments from there.  Rows are
+  // padded by 16 bytes, so the 16-byte fragment reads of a wave's lanes
+  // (one vector each) fall in distinct banks.
+  constexpr int CPV = KS * 2;                              // chunks per vector
+  constexpr int CHUNKS = 32 * CPV;                         // chunks per tile
+  constexpr int LPT = (CHUNKS + KNN_NT - 1) / KNN_NT;      // chunks per thread
+  constexpr int STRIDE = KS * 32 + 16;                     // LDS bytes per vector
+  __shared__ __attribute__((aligned(16))) unsigned char btile[2][32 * STRIDE];
+  // L2 filter: the tile's 32 squared norms travel with it (threads 0-31)
+  constexpr bool XN_TILE = (MODE == MODE_FILTER && METRIC == RK_KNN_L2);
+  __shared__ float xtile[XN_TILE ? 2 : 1][32];
+  const __amdgpu_buffer_rsrc_t rsx =
+      make_rsrc(XN_TILE ? (const void*)(a.xnorm + v0) : (const void*)X,
+                XN_TILE && v1 > v0 ? (uint32_t)((v1 - v0) * 4) : 0u);
+  float xr = 0.0f;
   const __amdgpu_buffer_rsrc_t rs =
       make_rsrc(X + v0 * a.ldx, v1 > v0 ? (uint32_t)(((v1 - v0 - 1) * a.ldx + a.d) * sizeof(U)) : 0u);
-  auto load_b = [&](uint4 (&bf)[KS], int64_t vt) {
-    const int64_t vb = vt + (lane & 31);
-    const bool vv = vb < v1;
-    const uint32_t base = (uint32_t)((vb - v0) * a.ldx * (int64_t)sizeof(U)) + (uint32_t)(lane >> 5) * 16u;
+  auto gload = [&](uint4 (&r)[LPT], int64_t vt) {
+#pragma unroll
+    for (int j = 0; j < LPT; ++j) {
+      const int c = (int)threadIdx.x + j * KNN_NT;
+      const int v = c / CPV, pc = c % CPV;
+      const int64_t vb = vt + v;
+      const bool ok = c < CHUNKS && vb < v1 && pc < nsteps * 2;
+      r[j] = bload16(rs, ok ? (uint32_t)((vb - v0) * a.ldx * (int64_t)sizeof(U)) + (uint32_t)pc * 16u
+                            : RSRC_OOB);
+    }
+    if constexpr (XN_TILE) {
+      const int64_t vb = vt + (int64_t)threadIdx.x;
+      xr = bload4f(rsx, (threadIdx.x < 32 && vb < v1) ? (uint32_t)((vb - v0) * 4) : RSRC_OOB);
+    }
+  };
+  auto lstore = [&](const uint4 (&r)[LPT], int buf) {
+#pragma unroll
+    for (int j = 0; j < LPT; ++j) {
+      const int c = (int)threadIdx.x + j * KNN_NT;
+      if (c < CHUNKS)
+        *reinterpret_cast<uint4*>(&btile[buf][(c / CPV) * STRIDE + (c % CPV) * 16]) = r[j];
+    }
+    if constexpr (XN_TILE)
+      if (threadIdx.x < 32) xtile[buf][threadIdx.x] = xr;
+  };
+  auto lfrag = [&](uint4 (&bf)[KS], int buf) {
 #pragma unroll
     for (int t = 0; t < KS; ++t)
-      bf[t] = bload16(rs, (t < nsteps && vv) ? base + (uint32_t)t * 32u : RSRC_OOB);
+      bf[t] = *reinterpret_cast<const uint4*>(&btile[buf][(lane & 31) * STRIDE + t * 32 + (lane >> 5) * 16]);
   };
   // Candidates of this wave wait in an LDS stage (written and read by this
   // wave only; LDS operations of one wave complete in order) and go to
@@ -228,7 +278,7 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     sn = 0;
   };
-  auto step = [&](const uint4 (&bf)[KS], int64_t vt) {
+  auto step = [&](const uint4 (&bf)[KS], int64_t vt, int buf) {
     const int64_t vb = vt + (lane & 31);
     const bool vv = vb < v1;
     f32x16 c[MT];
@@ -240,17 +290,21 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
     }
     float xn = 0.0f;
     if constexpr (METRIC == RK_KNN_L2) {
-      float xpart = 0.0f;
+      if constexpr (XN_TILE) {
+        xn = xtile[buf][lane & 31];
+      } else {
+        float xpart = 0.0f;
 #pragma unroll
-      for (int t = 0; t < KS; ++t) xpart = sq8<DT>(bf[t], xpart);
-      xn = __fadd_rn(xpart, __shfl_xor(xpart, 32, 64));
+        for (int t = 0; t < KS; ++t) xpart = sq8<DT>(bf[t], xpart);
+        xn = __fadd_rn(xpart, __shfl_xor(xpart, 32, 64));
+      }
     }
 #pragma unroll
     for (int m = 0; m < MT; ++m) {
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
         const int row = m * 32 + rowof(i);
-        const float v = (METRIC == RK_KNN_IP) ? c[m][i] : knn_value(METRIC, c[m][i], sh_qn[w][row], xn);
+        const float v = (METRIC == RK_KNN_IP) ? c[m][i] : knn_value(METRIC, c[m][i], qr[m][i], xn);
         if constexpr (MODE == MODE_SCORES) {
           const uint32_t q = sh_q[w][row];
           if (vv && q != 0xffffffffu) a.out[(size_t)q * a.ldo + vb] = v;
@@ -278,18 +332,60 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
   };
   // Two register buffers: the next step's loads are in flight while the
   // current step's MFMAs and compares run.
-  // The prefetches are unconditional (past v1 they read zeros, and a step
-  // there appends nothing), so no branch splits a load from its use.
-  uint4 b0[KS], b1[KS];
-  load_b(b0, v0);
-  for (int64_t vt = v0; vt < v1; vt += 64) {
-    load_b(b1, vt + 32);
-    step(b0, vt);
-    load_b(b0, vt + 64);
-    step(b1, vt + 32);
+  // Per step: fragments of tile s from one buffer, tile s+1 (in registers)
+  // to the other, loads of tile s+2 issued, then the MFMAs and compares of
+  // tile s; one barrier per step.  The loads past v1 read zeros and a step
+  // there appends nothing, so the trip count is the same in every wave.
+  uint4 r[LPT];
+  gload(r, v0);
+  lstore(r, 0);
+  gload(r, v0 + 32);
+  __syncthreads();
+  int buf = 0;
+  for (int64_t vt = v0; vt < v1; vt += 32) {
+    uint4 bf[KS];
+    lfrag(bf, buf);
+    lstore(r, buf ^ 1);
+    gload(r, vt + 64);
+    step(bf, vt, buf);
+    __syncthreads();
+    buf ^= 1;
   }
   if constexpr (MODE == MODE_FILTER)
     if (sn) flush();
+}
+
+// Squared norm of every database vector, with the operation order of the
+// score kernel's own (two halves of each 16-element step summed separately
+// in element order, then added), so that scores built from either are
+// bit-identical.
+template <int DT, int KS>
+__global__ __launch_bounds__(256) void k_knn_xnorm(const void* x, int64_t n, int32_t d, float* out) {
+  using U = typename Traits<DT>::U;
+  // two threads per vector, one per half; all of a half's loads in flight
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t j = g >> 1;
+  const int h = (int)(g & 1);
+  const bool ok = j < n;
+  const U* row = static_cast<const U*>(x) + (ok ? j : 0) * d + h * 8;
+  uint4 c[KS];
+#pragma unroll
+  for (int t = 0; t < KS; ++t)
+    c[t] = (ok && t < d / 16) ? ldg16<false>(row + t * 16) : make_uint4(0u, 0u, 0u, 0u);
+  float acc = 0.0f;
+#pragma unroll
+  for (int t = 0; t < KS; ++t) acc = sq8<DT>(c[t], acc);
+  const float other = __shfl_xor(acc, 1, 64);
+  if (ok && h == 0) out[j] = __fadd_rn(acc, other);
+}
+
+template <int DT>
+void launch_xnorm(const void* x, int64_t n, int32_t d, float* out, hipStream_t s) {
+  const unsigned nb = (unsigned)((2 * n + 255) / 256);
+  if (d <= 32) k_knn_xnorm<DT, 2><<<nb, 256, 0, s>>>(x, n, d, out);
+  else if (d <= 64) k_knn_xnorm<DT, 4><<<nb, 256, 0, s>>>(x, n, d, out);
+  else if (d <= 128) k_knn_xnorm<DT, 8><<<nb, 256, 0, s>>>(x, n, d, out);
+  else k_knn_xnorm<DT, 16><<<nb, 256, 0, s>>>(x, n, d, out);
 }
 
 // Per query: T_safe = the k-th best of the S sample keys, T = the r-th best.
@@ -425,7 +521,7 @@ __global__ __launch_bounds__(KNN_SEL_NT) void k_knn_select(int pass, const uint3
 }
 
 struct KnnLayout {
-  size_t hdr, T, T_safe, cnt, fail, big, total;
+  size_t hdr, T, T_safe, cnt, fail, xnorm, big, total;
   int32_t S;
   uint32_t cap;
 };
@@ -452,6 +548,7 @@ KnnLayout knn_layout(int32_t nq, int64_t n, int32_t k) {
   L.T_safe = o; o = a256(o + Q * 4);
   L.cnt = o; o = a256(o + Q * 4);
   L.fail = o; o = a256(o + Q * 4);
+  L.xnorm = o; o = a256(o + (size_t)n * 4);
   // the sample scores are dead once the thresholds exist: the candidate
   // slots reuse their bytes
   L.big = o;
@@ -604,6 +701,13 @@ extern "C" int rk_knn_topk(int dtype, int metric, const void* queries, const voi
   const int tgrid = std::min(nq, 4096);
   k_knn_threshold<<<tgrid, KNN_SEL_NT, 0, s>>>(sample, L.S, nq, n, k, metric, T, T_safe, cnt, hdr);
   if (hipGetLastError() != hipSuccess) return RK_E_HIP;
+  // squared database norms (L2), once per call
+  float* xnorm = reinterpret_cast<float*>(ws + L.xnorm);
+  if (metric == RK_KNN_L2) {
+    if (dtype == RK_BF16) launch_xnorm<BF16>(base, n, d, xnorm, s);
+    else launch_xnorm<F16>(base, n, d, xnorm, s);
+    if (hipGetLastError() != hipSuccess) return RK_E_HIP;
+  }
   // 3./5. fused filter; 4./6. select
   const int sgrid = std::min(nq, 8192);
   for (int pass = 0; pass < 2; ++pass) {
@@ -621,6 +725,7 @@ extern "C" int rk_knn_topk(int dtype, int metric, const void* queries, const voi
     f.cnt = cnt;
     f.cand = cand;
     f.cap = L.cap;
+    f.xnorm = metric == RK_KNN_L2 ? xnorm : nullptr;
     if (launch_knn_t<MODE_FILTER>(dtype, f, s) != hipSuccess) return RK_E_HIP;
     k_knn_select<<<sgrid, KNN_SEL_NT, dyn, s>>>(pass, pass ? fail : nullptr, pass ? nfail : nullptr,
                                                 nq, k, metric, (flags & RK_SORTED) ? 1 : 0, T, T_safe,
