@@ -319,6 +319,7 @@ def main() -> int:
         c_s = time.perf_counter() - c0
         result["cpu_baseline"] = {
             "value": float(s_off[-1]) / c_s, "unit": "keys/s", "cores": nth, "kind": "port",
+            "cpu_model": cpu_model(), "host_threads": orc.hardware_threads(),
             "sample": ("the whole workload" if stride == 1 else
                        f"every {stride}th row of the workload")
                       + f" ({len(rows)} rows, {int(s_off[-1])} keys), std::nth_element + "
@@ -422,6 +423,18 @@ def knn_bench(args) -> dict:
                        "achieved": out["inner_product"]["fused_tflops"],
                        "frac": out["inner_product"]["fused_frac_of_mfma_peak"]}
     return out
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (from /proc/cpuinfo), for the CPU baseline."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def percentile(x, q):

@@ -231,10 +231,9 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
   const __amdgpu_buffer_rsrc_t rsx =
       make_rsrc(XN_TILE ? (const void*)(a.xnorm + v0) : (const void*)X,
                 XN_TILE && v1 > v0 ? (uint32_t)((v1 - v0) * 4) : 0u);
-  float xr = 0.0f;
   const __amdgpu_buffer_rsrc_t rs =
       make_rsrc(X + v0 * a.ldx, v1 > v0 ? (uint32_t)(((v1 - v0 - 1) * a.ldx + a.d) * sizeof(U)) : 0u);
-  auto gload = [&](uint4 (&r)[LPT], int64_t vt) {
+  auto gload = [&](uint4 (&r)[LPT], float& xr, int64_t vt) {
 #pragma unroll
     for (int j = 0; j < LPT; ++j) {
       const int c = (int)threadIdx.x + j * KNN_NT;
@@ -249,7 +248,7 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
       xr = bload4f(rsx, (threadIdx.x < 32 && vb < v1) ? (uint32_t)((vb - v0) * 4) : RSRC_OOB);
     }
   };
-  auto lstore = [&](const uint4 (&r)[LPT], int buf) {
+  auto lstore = [&](const uint4 (&r)[LPT], float xr, int buf) {
 #pragma unroll
     for (int j = 0; j < LPT; ++j) {
       const int c = (int)threadIdx.x + j * KNN_NT;
@@ -332,24 +331,37 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
   };
   // Two register buffers: the next step's loads are in flight while the
   // current step's MFMAs and compares run.
-  // Per step: fragments of tile s from one buffer, tile s+1 (in registers)
-  // to the other, loads of tile s+2 issued, then the MFMAs and compares of
-  // tile s; one barrier per step.  The loads past v1 read zeros and a step
-  // there appends nothing, so the trip count is the same in every wave.
-  uint4 r[LPT];
-  gload(r, v0);
-  lstore(r, 0);
-  gload(r, v0 + 32);
+  // Per step: fragments of tile s from one LDS buffer, tile s+1 (loaded
+  // two steps earlier, in registers) to the other, loads of tile s+3
+  // issued, then the MFMAs and compares of tile s; one barrier per step.
+  // Two register sets alternate, so each tile's loads have two steps to
+  // arrive (a chunk's first reader on an XCD waits for HBM).  Loads past v1
+  // read zeros and a step there appends nothing, so every wave runs the same
+  // trip count (rounded up to an even number of steps).
+  uint4 ra[LPT], rb[LPT];
+  float xa = 0.0f, xb = 0.0f;
+  gload(ra, xa, v0);
+  lstore(ra, xa, 0);
+  gload(ra, xa, v0 + 32);
+  gload(rb, xb, v0 + 64);
   __syncthreads();
-  int buf = 0;
-  for (int64_t vt = v0; vt < v1; vt += 32) {
-    uint4 bf[KS];
-    lfrag(bf, buf);
-    lstore(r, buf ^ 1);
-    gload(r, vt + 64);
-    step(bf, vt, buf);
-    __syncthreads();
-    buf ^= 1;
+  for (int64_t vt = v0; vt < v1; vt += 64) {
+    {
+      uint4 bf[KS];
+      lfrag(bf, 0);
+      lstore(ra, xa, 1);
+      gload(ra, xa, vt + 96);
+      step(bf, vt, 0);
+      __syncthreads();
+    }
+    {
+      uint4 bf[KS];
+      lfrag(bf, 1);
+      lstore(rb, xb, 0);
+      gload(rb, xb, vt + 128);
+      step(bf, vt + 32, 1);
+      __syncthreads();
+    }
   }
   if constexpr (MODE == MODE_FILTER)
     if (sn) flush();
@@ -395,11 +407,14 @@ __global__ __launch_bounds__(KNN_SEL_NT) void k_knn_threshold(const float* sampl
                                                               int32_t metric, uint32_t* T,
                                                               uint32_t* T_safe, uint32_t* cnt,
                                                               Header* hdr) {
+  constexpr uint32_t TOP_CAP = 1024;  // keys >= T_safe kept for the second select
   __shared__ uint32_t keys[KNN_SAMPLE];
+  __shared__ uint32_t top[TOP_CAP];
   __shared__ uint32_t hist[HIST_MAX_BINS];
   __shared__ uint32_t scratch[16];
   __shared__ uint64_t scratch64[16];
   __shared__ uint32_t outv[4];
+  __shared__ uint32_t ntop;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     hdr->status = 0;
     hdr->fallback_rows = 0;
@@ -411,17 +426,30 @@ __global__ __launch_bounds__(KNN_SEL_NT) void k_knn_threshold(const float* sampl
   const float rf = ceilf(mu + 4.0f * sqrtf(mu) + 2.0f);
   const uint32_t r = (S == n || rf >= (float)k) ? (uint32_t)k : (uint32_t)rf;
   for (int q = blockIdx.x; q < nq; q += gridDim.x) {
-    const float* row = sample + (size_t)q * S;
-    for (int i = threadIdx.x; i < S; i += KNN_SEL_NT) {
-      const uint32_t bits = __float_as_uint(row[i]);
-      keys[i] = largest ? sel_key<F32, true>(bits) : sel_key<F32, false>(bits);
-    }
+    const uint32_t* row = reinterpret_cast<const uint32_t*>(sample) + (size_t)q * S;
+    load_to_lds<KNN_SEL_NT, 8>(keys, row, (uint32_t)S, [&](uint32_t bits, uint32_t) {
+      return largest ? sel_key<F32, true>(bits) : sel_key<F32, false>(bits);
+    });
+    if (threadIdx.x == 0) ntop = 0;
     __syncthreads();
     const uint32_t ts =
         block_radix_select<KNN_SEL_NT, 11, uint32_t, false>(keys, S, k, hist, scratch, scratch64, outv);
-    const uint32_t tp = r < (uint32_t)k
-        ? block_radix_select<KNN_SEL_NT, 11, uint32_t, false>(keys, S, r, hist, scratch, scratch64, outv)
-        : ts;
+    uint32_t tp = ts;
+    if (r < (uint32_t)k) {
+      // T >= T_safe, so the r-th best key is the r-th best of the keys >= T_safe
+      for (int i = threadIdx.x; i < S; i += KNN_SEL_NT) {
+        const uint32_t x = keys[i];
+        if (x >= ts) {
+          const uint32_t pos = atomicAdd(&ntop, 1u);
+          if (pos < TOP_CAP) top[pos] = x;
+        }
+      }
+      __syncthreads();
+      const uint32_t nt = ntop;
+      tp = nt <= TOP_CAP
+          ? block_radix_select<KNN_SEL_NT, 11, uint32_t, false>(top, nt, r, hist, scratch, scratch64, outv)
+          : block_radix_select<KNN_SEL_NT, 11, uint32_t, false>(keys, S, r, hist, scratch, scratch64, outv);
+    }
     if (threadIdx.x == 0) {
       T_safe[q] = ts;
       T[q] = tp;
