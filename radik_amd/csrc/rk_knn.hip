@@ -298,21 +298,41 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
         xn = __fadd_rn(xpart, __shfl_xor(xpart, 32, 64));
       }
     }
+    if constexpr (MODE == MODE_SCORES) {
 #pragma unroll
-    for (int m = 0; m < MT; ++m) {
+      for (int m = 0; m < MT; ++m)
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int row = m * 32 + rowof(i);
-        const float v = (METRIC == RK_KNN_IP) ? c[m][i] : knn_value(METRIC, c[m][i], qr[m][i], xn);
-        if constexpr (MODE == MODE_SCORES) {
+        for (int i = 0; i < 16; ++i) {
+          const int row = m * 32 + rowof(i);
+          const float v = (METRIC == RK_KNN_IP) ? c[m][i] : knn_value(METRIC, c[m][i], qr[m][i], xn);
           const uint32_t q = sh_q[w][row];
           if (vv && q != 0xffffffffu) a.out[(size_t)q * a.ldo + vb] = v;
-        } else {
-          const bool pass = LARGEST ? !(v < tf[m][i]) : !(v > tf[m][i]);
-          if (__ballot(pass)) {  // wave-uniform: some lane may hold a candidate
+        }
+    } else {
+      // Compares in groups of four accumulators: one wave-uniform branch per
+      // group; the exact check runs only in a group where some lane passed.
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int g = 0; g < 16; g += 4) {
+          float v[4];
+          bool pass[4];
+          uint64_t any = 0;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int i = g + e;
+            v[e] = (METRIC == RK_KNN_IP) ? c[m][i] : knn_value(METRIC, c[m][i], qr[m][i], xn);
+            pass[e] = LARGEST ? !(v[e] < tf[m][i]) : !(v[e] > tf[m][i]);
+            any |= __ballot(pass[e]);
+          }
+          if (!any) continue;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            if (!__ballot(pass[e])) continue;
+            const int row = m * 32 + rowof(g + e);
             const uint32_t q = sh_q[w][row];
-            const uint32_t key = sel_key<F32, LARGEST>(__float_as_uint(v));
-            const bool take = pass && vv && q != 0xffffffffu && key >= sh_t[w][row];
+            const uint32_t key = sel_key<F32, LARGEST>(__float_as_uint(v[e]));
+            const bool take = pass[e] && vv && q != 0xffffffffu && key >= sh_t[w][row];
             const uint64_t tm = __ballot(take);
             if (tm) {
               const uint32_t nt = (uint32_t)__popcll(tm);
@@ -326,11 +346,8 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
             }
           }
         }
-      }
     }
   };
-  // Two register buffers: the next step's loads are in flight while the
-  // current step's MFMAs and compares run.
   // Per step: fragments of tile s from one LDS buffer, tile s+1 (loaded
   // two steps earlier, in registers) to the other, loads of tile s+3
   // issued, then the MFMAs and compares of tile s; one barrier per step.
