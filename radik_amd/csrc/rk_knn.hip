@@ -277,16 +277,17 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     sn = 0;
   };
-  auto step = [&](const uint4 (&bf)[KS], int64_t vt, int buf) {
-    const int64_t vb = vt + (lane & 31);
-    const bool vv = vb < v1;
-    f32x16 c[MT];
+  // scores of one tile: MT accumulator sets
+  auto mma = [&](const uint4 (&bf)[KS], f32x16 (&c)[MT]) {
 #pragma unroll
     for (int m = 0; m < MT; ++m) {
       c[m] = f32x16{};
 #pragma unroll
       for (int t = 0; t < KS; ++t) c[m] = Mfma<DT>::run(af[m][t], bf[t], c[m]);
     }
+  };
+  // squared norm of this lane's vector of the tile (L2)
+  auto xn_of = [&](const uint4 (&bf)[KS], int buf) -> float {
     float xn = 0.0f;
     if constexpr (METRIC == RK_KNN_L2) {
       if constexpr (XN_TILE) {
@@ -298,6 +299,12 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
         xn = __fadd_rn(xpart, __shfl_xor(xpart, 32, 64));
       }
     }
+    return xn;
+  };
+  // compares (filter) or stores (scores) of the tile at vt
+  auto cmp = [&](const f32x16 (&c)[MT], float xn, int64_t vt) {
+    const int64_t vb = vt + (lane & 31);
+    const bool vv = vb < v1;
     if constexpr (MODE == MODE_SCORES) {
 #pragma unroll
       for (int m = 0; m < MT; ++m)
@@ -355,6 +362,16 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
   // arrive (a chunk's first reader on an XCD waits for HBM).  Loads past v1
   // read zeros and a step there appends nothing, so every wave runs the same
   // trip count (rounded up to an even number of steps).
+  auto body = [&](uint4 (&R)[LPT], float& xr, int buf, int64_t vt) {
+    uint4 bf[KS];
+    lfrag(bf, buf);
+    lstore(R, xr, buf ^ 1);
+    gload(R, xr, vt + 96);
+    f32x16 c[MT];
+    mma(bf, c);
+    cmp(c, xn_of(bf, buf), vt);
+    __syncthreads();
+  };
   uint4 ra[LPT], rb[LPT];
   float xa = 0.0f, xb = 0.0f;
   gload(ra, xa, v0);
@@ -363,22 +380,8 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
   gload(rb, xb, v0 + 64);
   __syncthreads();
   for (int64_t vt = v0; vt < v1; vt += 64) {
-    {
-      uint4 bf[KS];
-      lfrag(bf, 0);
-      lstore(ra, xa, 1);
-      gload(ra, xa, vt + 96);
-      step(bf, vt, 0);
-      __syncthreads();
-    }
-    {
-      uint4 bf[KS];
-      lfrag(bf, 1);
-      lstore(rb, xb, 0);
-      gload(rb, xb, vt + 128);
-      step(bf, vt + 32, 1);
-      __syncthreads();
-    }
+    body(ra, xa, 0, vt);
+    body(rb, xb, 1, vt + 32);
   }
   if constexpr (MODE == MODE_FILTER)
     if (sn) flush();
