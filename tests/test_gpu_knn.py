@@ -11,6 +11,8 @@ against a per-query threshold inside the score kernel, then selected exactly.
 - The second pass: a database whose sampled rows hold the best scores makes
   the first threshold admit fewer than k candidates.
 - Candidate overflow (every score tied) is reported in the status word.
+- hipGraph capture: replays equal the eager result, also after the queries
+  change in place.
 """
 
 from __future__ import annotations
@@ -150,3 +152,45 @@ def test_arguments_rejected():
         rk.knn_topk(q.float(), x.float(), 5)  # fp32 inputs are not MFMA inputs here
     with pytest.raises(rk.RadikError):
         rk.knn_topk(q[:, :40].contiguous(), x[:, :40].contiguous(), 5)  # d % 16
+
+
+@pytest.mark.parametrize("metric", [0, 1])
+def test_graph_replays(metric):
+    """rk_knn_topk captured in a hipGraph: every replay (with eager calls on
+    other buffers and synchronizations in between) equals the eager result,
+    and a replay after the queries change in place equals the eager result
+    on the new queries."""
+    import torch
+
+    import radik_amd as rk
+    q, x = _data(200, 30_000, 64, "bf16", 21)
+    k = 50
+    ref_v, ref_i, _ = rk.knn_topk(q, x, k, metric=metric)
+    torch.cuda.synchronize()
+    ws = torch.empty(rk.knn_workspace_size(rk.RK_BF16, 200, 30_000, 64, k), dtype=torch.uint8,
+                     device="cuda")
+    out_v = torch.empty_like(ref_v)
+    out_i = torch.empty_like(ref_i)
+    s = torch.cuda.Stream()
+    with rk.Graph.capture(s) as g:
+        rk.knn_topk(q, x, k, metric=metric, workspace=ws, out_vals=out_v, out_idx=out_i, stream=s)
+    for rep in range(3):
+        out_v.fill_(0)
+        out_i.fill_(-5)
+        torch.cuda.synchronize()
+        g.launch(s)
+        s.synchronize()
+        assert rk.status(ws)[0] == 0
+        assert torch.equal(out_i, ref_i), f"replay {rep}: indices differ"
+        assert torch.equal(out_v.view(torch.int32), ref_v.view(torch.int32)), f"replay {rep}"
+        ev, ei, _ = rk.knn_topk(q, x, k, metric=metric)  # eager, own buffers
+        torch.cuda.synchronize()
+        assert torch.equal(ei, ref_i)
+    q.copy_(torch.flip(q, dims=[0]))
+    new_v, new_i, _ = rk.knn_topk(q, x, k, metric=metric)
+    torch.cuda.synchronize()
+    g.launch(s)
+    s.synchronize()
+    assert torch.equal(out_i, new_i)
+    assert torch.equal(out_v.view(torch.int32), new_v.view(torch.int32))
+    g.close()
