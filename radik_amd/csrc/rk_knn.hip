@@ -220,14 +220,19 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
   // buffers, and every wave reads its B fragments from there.  Rows are
   // padded by 16 bytes, so the 16-byte fragment reads of a wave's lanes
   // (one vector each) fall in distinct banks.
+  // A tile is SUB groups of 32 vectors: two for the inner-product filter at
+  // d <= 128, where the registers allow the larger staging (half the
+  // barriers per MFMA), one elsewhere.
+  constexpr int SUB = (MODE == MODE_FILTER && METRIC == RK_KNN_IP && KS <= 8) ? 2 : 1;
+  constexpr int TV = 32 * SUB;                             // vectors per tile
   constexpr int CPV = KS * 2;                              // chunks per vector
-  constexpr int CHUNKS = 32 * CPV;                         // chunks per tile
+  constexpr int CHUNKS = TV * CPV;                         // chunks per tile
   constexpr int LPT = (CHUNKS + KNN_NT - 1) / KNN_NT;      // chunks per thread
   constexpr int STRIDE = KS * 32 + 16;                     // LDS bytes per vector
-  __shared__ __attribute__((aligned(16))) unsigned char btile[2][32 * STRIDE];
+  __shared__ __attribute__((aligned(16))) unsigned char btile[2][TV * STRIDE];
   // L2 filter: the tile's 32 squared norms travel with it (threads 0-31)
   constexpr bool XN_TILE = (MODE == MODE_FILTER && METRIC == RK_KNN_L2);
-  __shared__ float xtile[XN_TILE ? 2 : 1][32];
+  __shared__ float xtile[XN_TILE ? 2 : 1][TV];
   const __amdgpu_buffer_rsrc_t rsx =
       make_rsrc(XN_TILE ? (const void*)(a.xnorm + v0) : (const void*)X,
                 XN_TILE && v1 > v0 ? (uint32_t)((v1 - v0) * 4) : 0u);
@@ -245,7 +250,7 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
     }
     if constexpr (XN_TILE) {
       const int64_t vb = vt + (int64_t)threadIdx.x;
-      xr = bload4f(rsx, (threadIdx.x < 32 && vb < v1) ? (uint32_t)((vb - v0) * 4) : RSRC_OOB);
+      xr = bload4f(rsx, (threadIdx.x < TV && vb < v1) ? (uint32_t)((vb - v0) * 4) : RSRC_OOB);
     }
   };
   auto lstore = [&](const uint4 (&r)[LPT], float xr, int buf) {
@@ -256,12 +261,13 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
         *reinterpret_cast<uint4*>(&btile[buf][(c / CPV) * STRIDE + (c % CPV) * 16]) = r[j];
     }
     if constexpr (XN_TILE)
-      if (threadIdx.x < 32) xtile[buf][threadIdx.x] = xr;
+      if (threadIdx.x < TV) xtile[buf][threadIdx.x] = xr;
   };
-  auto lfrag = [&](uint4 (&bf)[KS], int buf) {
+  auto lfrag = [&](uint4 (&bf)[KS], int buf, int sub) {
 #pragma unroll
     for (int t = 0; t < KS; ++t)
-      bf[t] = *reinterpret_cast<const uint4*>(&btile[buf][(lane & 31) * STRIDE + t * 32 + (lane >> 5) * 16]);
+      bf[t] = *reinterpret_cast<const uint4*>(
+          &btile[buf][(sub * 32 + (lane & 31)) * STRIDE + t * 32 + (lane >> 5) * 16]);
   };
   // Candidates of this wave wait in an LDS stage (written and read by this
   // wave only; LDS operations of one wave complete in order) and go to
@@ -287,11 +293,11 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
     }
   };
   // squared norm of this lane's vector of the tile (L2)
-  auto xn_of = [&](const uint4 (&bf)[KS], int buf) -> float {
+  auto xn_of = [&](const uint4 (&bf)[KS], int buf, int sub) -> float {
     float xn = 0.0f;
     if constexpr (METRIC == RK_KNN_L2) {
       if constexpr (XN_TILE) {
-        xn = xtile[buf][lane & 31];
+        xn = xtile[buf][sub * 32 + (lane & 31)];
       } else {
         float xpart = 0.0f;
 #pragma unroll
@@ -357,31 +363,37 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
   };
   // Per step: fragments of tile s from one LDS buffer, tile s+1 (loaded
   // two steps earlier, in registers) to the other, loads of tile s+3
-  // issued, then the MFMAs and compares of tile s; one barrier per step.
+  // issued, then the MFMAs and compares of tile s, one group of 32 vectors
+  // at a time; one barrier per step.
   // Two register sets alternate, so each tile's loads have two steps to
   // arrive (a chunk's first reader on an XCD waits for HBM).  Loads past v1
   // read zeros and a step there appends nothing, so every wave runs the same
   // trip count (rounded up to an even number of steps).
   auto body = [&](uint4 (&R)[LPT], float& xr, int buf, int64_t vt) {
-    uint4 bf[KS];
-    lfrag(bf, buf);
-    lstore(R, xr, buf ^ 1);
-    gload(R, xr, vt + 96);
-    f32x16 c[MT];
-    mma(bf, c);
-    cmp(c, xn_of(bf, buf), vt);
+#pragma unroll
+    for (int sub = 0; sub < SUB; ++sub) {
+      uint4 bf[KS];
+      lfrag(bf, buf, sub);
+      if (sub == 0) {
+        lstore(R, xr, buf ^ 1);
+        gload(R, xr, vt + 3 * TV);
+      }
+      f32x16 c[MT];
+      mma(bf, c);
+      cmp(c, xn_of(bf, buf, sub), vt + sub * 32);
+    }
     __syncthreads();
   };
   uint4 ra[LPT], rb[LPT];
   float xa = 0.0f, xb = 0.0f;
   gload(ra, xa, v0);
   lstore(ra, xa, 0);
-  gload(ra, xa, v0 + 32);
-  gload(rb, xb, v0 + 64);
+  gload(ra, xa, v0 + TV);
+  gload(rb, xb, v0 + 2 * TV);
   __syncthreads();
-  for (int64_t vt = v0; vt < v1; vt += 64) {
+  for (int64_t vt = v0; vt < v1; vt += 2 * TV) {
     body(ra, xa, 0, vt);
-    body(rb, xb, 1, vt + 32);
+    body(rb, xb, 1, vt + TV);
   }
   if constexpr (MODE == MODE_FILTER)
     if (sn) flush();
