@@ -94,8 +94,8 @@ def main() -> int:
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; rows are independent and "
-                          "the multi-GPU design (row sharding) is described in DESIGN.md",
+        print(json.dumps({"error": "this build runs on one GPU only; rows are independent, so "
+                          "a batch would shard across GPUs by rows with no exchange",
                           "n_gpus": args.gpus}))
         return 2
 

@@ -84,7 +84,7 @@ Layout make_layout(int32_t R, int64_t total_len, int32_t max_k) {
   L.tile_start = o; o = align256(o + (Ru + 1) * sizeof(uint32_t));
   L.cand_base = o; o = align256(o + (Ru + 1) * sizeof(uint64_t));
   L.cand_cnt = o; o = align256(o + Ru * sizeof(uint32_t));
-  L.order = o; o = align256(o + Ru * sizeof(uint32_t));
+  L.order = o; o = align256(o + 2 * Ru * sizeof(uint32_t));
   L.ghist = o; o = align256(o + Ru * rk::HIST_MAX_BINS * sizeof(uint32_t));
   L.tile_cnt = o; o = align256(o + L.tile_cap * sizeof(uint2));
   L.tile_pre = o; o = align256(o + L.tile_cap * sizeof(uint2));

@@ -381,29 +381,41 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
     if (status & (ST_TILES | ST_CANDIDATES)) p.tile_start[p.R] = 0;  // disable streaming
   }
   __syncthreads();
-  // Row order of the per-row kernels: counting sort by floor(log2 k),
-  // largest first, so the longest selections and sorts start first.
+  // Row orders of the per-row kernels, largest first, by counting sorts:
+  // order[0, R) by floor(log2 k) for the select (the longest selections and
+  // sorts start first), order[R, 2R) by floor(log2 len) for the sample,
+  // whose cost grows with the row length.
   {
-    __shared__ uint32_t bucket[16];
-    if (tid < 16) bucket[tid] = 0;
+    __shared__ uint32_t bucket[2][32];
+    if (tid < 64) bucket[tid >> 5][tid & 31] = 0;
     __syncthreads();
-    for (int r = tid; r < p.R; r += 1024) {
+    auto classes = [&](int r, uint32_t& kc, uint32_t& lc) {
       const uint32_t k = (uint32_t)max(p.desc[r].k, 1);
-      atomicAdd(&bucket[min(31 - __builtin_clz(k), 15)], 1u);
+      const uint64_t len = (uint64_t)max<int64_t>(p.desc[r].len, 1);
+      kc = (uint32_t)min(31 - __builtin_clz(k), 15);
+      lc = (uint32_t)min(63 - __builtin_clzll(len), 31);
+    };
+    for (int r = tid; r < p.R; r += 1024) {
+      uint32_t kc, lc;
+      classes(r, kc, lc);
+      atomicAdd(&bucket[0][kc], 1u);
+      atomicAdd(&bucket[1][lc], 1u);
     }
     __syncthreads();
-    if (tid == 0) {
+    if (tid < 2) {
       uint32_t acc = 0;
-      for (int c = 15; c >= 0; --c) {
-        const uint32_t n = bucket[c];
-        bucket[c] = acc;
+      for (int c = 31; c >= 0; --c) {
+        const uint32_t n = bucket[tid][c];
+        bucket[tid][c] = acc;
         acc += n;
       }
     }
     __syncthreads();
     for (int r = tid; r < p.R; r += 1024) {
-      const uint32_t k = (uint32_t)max(p.desc[r].k, 1);
-      p.order[atomicAdd(&bucket[min(31 - __builtin_clz(k), 15)], 1u)] = (uint32_t)r;
+      uint32_t kc, lc;
+      classes(r, kc, lc);
+      p.order[atomicAdd(&bucket[0][kc], 1u)] = (uint32_t)r;
+      p.order[p.R + atomicAdd(&bucket[1][lc], 1u)] = (uint32_t)r;
     }
   }
   // On a capacity overflow no long row is processed (the status reports it).
@@ -450,7 +462,7 @@ __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
     const uint32_t qi = next_sh;
     __syncthreads();
     if (qi >= (uint32_t)p.R) break;
-    const int r = (int)p.order[qi];
+    const int r = (int)p.order[p.R + qi];
     if (p.st[r].phase != PH_SAMPLE) continue;
     record_chunk_rows<SEL_NT>(p, r);
     RK_STAMP(p, r, 0);

@@ -111,7 +111,8 @@ struct KParams {
   uint32_t* tile_start;  // [R + 1]
   uint64_t* cand_base;   // [R + 1]
   uint32_t* cand_cnt;    // [R]
-  uint32_t* order;       // [R] rows by descending cost class (largest k first)
+  uint32_t* order;       // [2R] rows by descending cost class: largest k first
+                         // (select), then longest first (sample)
   uint32_t* ghist;       // [R][2048]
   uint2* tile_cnt;       // [tile_capacity]
   uint2* tile_pre;       // [tile_capacity]
