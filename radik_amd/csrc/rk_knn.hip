@@ -130,10 +130,11 @@ RK_DEV float knn_value(int metric, float dot, float qn, float xn) {
 // threshold, held in registers: !(v < tf) for largest-first (!(v > tf) for
 // smallest-first) is true for every score whose key reaches T (tf is the
 // float of T, NaN when T = 0, so that everything passes), for NaN scores, and
-// for +-0 next to a zero threshold.  Only when some lane of the wave passes
-// (a wave-uniform branch) is the exact key compared and the candidate
-// appended; with thresholds near the k-th best score that is a small
-// fraction of the compares.
+// for +-0 next to a zero threshold.  The compares go in groups of four
+// accumulators with one wave-uniform branch per group: only when some lane
+// passes one of them is the exact key compared and the candidate staged;
+// with thresholds near the k-th best score that is a small fraction of the
+// groups.
 template <int DT, int KS, int MT, int MODE, int METRIC>
 __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
   using U = typename Traits<DT>::U;
@@ -213,7 +214,7 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
   }
   const int64_t v0 = (int64_t)chunk * a.chunk_v;
   const int64_t v1 = std::min<int64_t>(a.nv, v0 + a.chunk_v);
-  // B tiles (32 vectors x d) go through LDS once per workgroup: the 256
+  // B tiles (TV vectors x d) go through LDS once per workgroup: the 256
   // threads load a tile's 16-byte chunks with buffer loads from this chunk's
   // range (vectors past v1 and chunks past d read zeros without a branch;
   // the host keeps a chunk's bytes below 2^32), store them to one of two LDS
@@ -230,7 +231,7 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
   constexpr int LPT = (CHUNKS + KNN_NT - 1) / KNN_NT;      // chunks per thread
   constexpr int STRIDE = KS * 32 + 16;                     // LDS bytes per vector
   __shared__ __attribute__((aligned(16))) unsigned char btile[2][TV * STRIDE];
-  // L2 filter: the tile's 32 squared norms travel with it (threads 0-31)
+  // L2 filter: the tile's TV squared norms travel with it (threads 0..TV-1)
   constexpr bool XN_TILE = (MODE == MODE_FILTER && METRIC == RK_KNN_L2);
   __shared__ float xtile[XN_TILE ? 2 : 1][TV];
   const __amdgpu_buffer_rsrc_t rsx =
