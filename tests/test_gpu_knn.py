@@ -194,3 +194,19 @@ def test_graph_replays(metric):
     assert torch.equal(out_i, new_i)
     assert torch.equal(out_v.view(torch.int32), new_v.view(torch.int32))
     g.close()
+
+
+@pytest.mark.parametrize("nq,n,k", [(1, 50, 50), (2, 31, 7), (1, 16385, 1), (3, 100_000, 16384)])
+def test_tiny_query_counts_and_k_equal_n(nq, n, k):
+    """One or two queries, databases below one 32-vector tile, k = n, and the
+    largest k: fused equals unfused for both metrics and both orders."""
+    import radik_amd as rk
+    q, x = _data(nq, n, 32, "f16", nq * 7 + n)
+    for metric in (rk.RK_KNN_IP, rk.RK_KNN_L2):
+        for sorted_ in (True, False):
+            v, i, ws = rk.knn_topk(q, x, k, metric=metric, sorted=sorted_)
+            assert rk.status(ws)[0] == 0
+            _, uv, ui = _unfused(q, x, k, metric, sorted_)
+            np.testing.assert_array_equal(i.cpu().numpy(), ui.cpu().numpy())
+            np.testing.assert_array_equal(v.cpu().numpy().view(np.uint32),
+                                          uv.cpu().numpy().view(np.uint32))
