@@ -197,20 +197,31 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
 #pragma unroll
       for (int i = 0; i < 16; ++i) qr[m][i] = sh_qn[w][m * 32 + rowof(i)];
   }
-  // per-accumulator float thresholds (filter mode)
+  // Per-row float thresholds (filter mode).  The inner-product filter keeps
+  // them in LDS (the four rows of a compare group are consecutive: one
+  // 16-byte load per group), which with 32-vector tiles and one prefetch set
+  // brings it to 126 registers, 4 waves per SIMD (~4 % faster than 3 in an
+  // A/B); L2, at 3 waves per SIMD either way, keeps them in registers.
+  constexpr bool LEAN = (MODE == MODE_FILTER && METRIC == RK_KNN_IP);
+  __shared__ __attribute__((aligned(16))) float sh_tf[LEAN ? NW : 1][LEAN ? QW : 4];
   float tf[MT][16];
   if constexpr (MODE == MODE_FILTER) {
+    auto tf_of = [&](int row) {
+      const uint32_t T = sh_t[w][row];
+      const float inf = __builtin_inff();
+      return sh_q[w][row] == 0xffffffffu ? (LARGEST ? inf : -inf)
+           : T == 0u ? __builtin_nanf("")
+           : __uint_as_float(key_to_bits<F32, LARGEST>(T));
+    };
+    if constexpr (LEAN) {
+      if (lane < QW) sh_tf[w][lane] = tf_of(lane);
+      __syncthreads();
+    } else {
 #pragma unroll
-    for (int m = 0; m < MT; ++m)
+      for (int m = 0; m < MT; ++m)
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int row = m * 32 + rowof(i);
-        const uint32_t T = sh_t[w][row];
-        const float inf = __builtin_inff();
-        tf[m][i] = sh_q[w][row] == 0xffffffffu ? (LARGEST ? inf : -inf)
-                 : T == 0u ? __builtin_nanf("")
-                 : __uint_as_float(key_to_bits<F32, LARGEST>(T));
-      }
+        for (int i = 0; i < 16; ++i) tf[m][i] = tf_of(m * 32 + rowof(i));
+    }
   }
   const int64_t v0 = (int64_t)chunk * a.chunk_v;
   const int64_t v1 = std::min<int64_t>(a.nv, v0 + a.chunk_v);
@@ -221,11 +232,7 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
   // buffers, and every wave reads its B fragments from there.  Rows are
   // padded by 16 bytes, so the 16-byte fragment reads of a wave's lanes
   // (one vector each) fall in distinct banks.
-  // A tile is SUB groups of 32 vectors: two for the inner-product filter at
-  // d <= 128, where the registers allow the larger staging (half the
-  // barriers per MFMA), one elsewhere.
-  constexpr int SUB = (MODE == MODE_FILTER && METRIC == RK_KNN_IP && KS <= 8) ? 2 : 1;
-  constexpr int TV = 32 * SUB;                             // vectors per tile
+  constexpr int TV = 32;                                   // vectors per tile
   constexpr int CPV = KS * 2;                              // chunks per vector
   constexpr int CHUNKS = TV * CPV;                         // chunks per tile
   constexpr int LPT = (CHUNKS + KNN_NT - 1) / KNN_NT;      // chunks per thread
@@ -264,11 +271,10 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
     if constexpr (XN_TILE)
       if (threadIdx.x < TV) xtile[buf][threadIdx.x] = xr;
   };
-  auto lfrag = [&](uint4 (&bf)[KS], int buf, int sub) {
+  auto lfrag = [&](uint4 (&bf)[KS], int buf) {
 #pragma unroll
     for (int t = 0; t < KS; ++t)
-      bf[t] = *reinterpret_cast<const uint4*>(
-          &btile[buf][(sub * 32 + (lane & 31)) * STRIDE + t * 32 + (lane >> 5) * 16]);
+      bf[t] = *reinterpret_cast<const uint4*>(&btile[buf][(lane & 31) * STRIDE + t * 32 + (lane >> 5) * 16]);
   };
   // Candidates of this wave wait in an LDS stage (written and read by this
   // wave only; LDS operations of one wave complete in order) and go to
@@ -294,11 +300,11 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
     }
   };
   // squared norm of this lane's vector of the tile (L2)
-  auto xn_of = [&](const uint4 (&bf)[KS], int buf, int sub) -> float {
+  auto xn_of = [&](const uint4 (&bf)[KS], int buf) -> float {
     float xn = 0.0f;
     if constexpr (METRIC == RK_KNN_L2) {
       if constexpr (XN_TILE) {
-        xn = xtile[buf][sub * 32 + (lane & 31)];
+        xn = xtile[buf][lane & 31];
       } else {
         float xpart = 0.0f;
 #pragma unroll
@@ -329,14 +335,21 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
       for (int m = 0; m < MT; ++m)
 #pragma unroll
         for (int g = 0; g < 16; g += 4) {
-          float v[4];
+          float v[4], tg[4];
           bool pass[4];
           uint64_t any = 0;
+          if constexpr (LEAN) {
+            const float4 t4 = *reinterpret_cast<const float4*>(&sh_tf[w][m * 32 + rowof(g)]);
+            tg[0] = t4.x; tg[1] = t4.y; tg[2] = t4.z; tg[3] = t4.w;
+          } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) tg[e] = tf[m][g + e];
+          }
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             const int i = g + e;
             v[e] = (METRIC == RK_KNN_IP) ? c[m][i] : knn_value(METRIC, c[m][i], qr[m][i], xn);
-            pass[e] = LARGEST ? !(v[e] < tf[m][i]) : !(v[e] > tf[m][i]);
+            pass[e] = LARGEST ? !(v[e] < tg[e]) : !(v[e] > tg[e]);
             any |= __ballot(pass[e]);
           }
           if (!any) continue;
@@ -362,39 +375,34 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
         }
     }
   };
-  // Per step: fragments of tile s from one LDS buffer, tile s+1 (loaded
-  // two steps earlier, in registers) to the other, loads of tile s+3
-  // issued, then the MFMAs and compares of tile s, one group of 32 vectors
-  // at a time; one barrier per step.
-  // Two register sets alternate, so each tile's loads have two steps to
-  // arrive (a chunk's first reader on an XCD waits for HBM).  Loads past v1
-  // read zeros and a step there appends nothing, so every wave runs the same
-  // trip count (rounded up to an even number of steps).
+  // Per step: fragments of tile s from one LDS buffer, tile s+1 (in
+  // registers) to the other, loads of a later tile issued, then the MFMAs and
+  // compares of tile s; one barrier per step.  With two register sets each
+  // tile's loads have two steps to arrive (a chunk's first reader on an XCD
+  // waits for HBM); the lean inner-product filter uses one set (one step).
+  // Loads past v1 read zeros and a step there appends nothing, so every
+  // wave runs the same trip count (rounded up to an even number of steps).
+  constexpr int NSET = LEAN ? 1 : 2;
   auto body = [&](uint4 (&R)[LPT], float& xr, int buf, int64_t vt) {
-#pragma unroll
-    for (int sub = 0; sub < SUB; ++sub) {
-      uint4 bf[KS];
-      lfrag(bf, buf, sub);
-      if (sub == 0) {
-        lstore(R, xr, buf ^ 1);
-        gload(R, xr, vt + 3 * TV);
-      }
-      f32x16 c[MT];
-      mma(bf, c);
-      cmp(c, xn_of(bf, buf, sub), vt + sub * 32);
-    }
+    uint4 bf[KS];
+    lfrag(bf, buf);
+    lstore(R, xr, buf ^ 1);
+    gload(R, xr, vt + (NSET + 1) * TV);
+    f32x16 c[MT];
+    mma(bf, c);
+    cmp(c, xn_of(bf, buf), vt);
     __syncthreads();
   };
-  uint4 ra[LPT], rb[LPT];
-  float xa = 0.0f, xb = 0.0f;
-  gload(ra, xa, v0);
-  lstore(ra, xa, 0);
-  gload(ra, xa, v0 + TV);
-  gload(rb, xb, v0 + 2 * TV);
+  uint4 rr[NSET][LPT];
+  float xx[NSET] = {};
+  gload(rr[0], xx[0], v0);
+  lstore(rr[0], xx[0], 0);
+#pragma unroll
+  for (int j = 0; j < NSET; ++j) gload(rr[j], xx[j], v0 + (j + 1) * TV);
   __syncthreads();
   for (int64_t vt = v0; vt < v1; vt += 2 * TV) {
-    body(ra, xa, 0, vt);
-    body(rb, xb, 1, vt + TV);
+    body(rr[0], xx[0], 0, vt);
+    body(rr[NSET - 1], xx[NSET - 1], 1, vt + TV);
   }
   if constexpr (MODE == MODE_FILTER)
     if (sn) flush();
