@@ -136,12 +136,12 @@ RK_DEV float knn_value(int metric, float dot, float qn, float xn) {
 // with thresholds near the k-th best score that is a small fraction of the
 // groups.
 template <int DT, int KS, int MT, int MODE, int METRIC>
-__global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
+__global__ __launch_bounds__(KNN_NT) __attribute__((amdgpu_waves_per_eu(MODE == MODE_FILTER && KS <= 8 ? 4 : 1))) void k_knn(KnnArgs a) {
   using U = typename Traits<DT>::U;
   constexpr int NW = KNN_NT / 64;
   constexpr int QW = 32 * MT;  // queries per wave
   constexpr bool LARGEST = (METRIC == RK_KNN_IP);
-  __shared__ float sh_qn[NW][QW];
+  __shared__ __attribute__((aligned(16))) float sh_qn[NW][QW];
   __shared__ uint32_t sh_t[NW][QW];
   __shared__ uint32_t sh_q[NW][QW];
   __shared__ uint64_t st_key[MODE == MODE_FILTER ? NW : 1][KNN_STAGE];
@@ -189,9 +189,10 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
   __syncthreads();
   // accumulator i of this lane belongs to row rowof(i) of a 32-query tile
   auto rowof = [&](int i) { return 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3); };
-  // per-accumulator query norms (L2)
+  // per-accumulator query norms (L2; the lean filter reads them from LDS
+  // per compare group, like its thresholds)
   float qr[MT][16];
-  if constexpr (METRIC == RK_KNN_L2) {
+  if constexpr (METRIC == RK_KNN_L2 && MODE != MODE_FILTER) {
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
@@ -202,7 +203,7 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
   // 16-byte load per group), which with 32-vector tiles and one prefetch set
   // brings it to 126 registers, 4 waves per SIMD (~4 % faster than 3 in an
   // A/B); L2, at 3 waves per SIMD either way, keeps them in registers.
-  constexpr bool LEAN = (MODE == MODE_FILTER && METRIC == RK_KNN_IP);
+  constexpr bool LEAN = (MODE == MODE_FILTER);
   __shared__ __attribute__((aligned(16))) float sh_tf[LEAN ? NW : 1][LEAN ? QW : 4];
   float tf[MT][16];
   if constexpr (MODE == MODE_FILTER) {
@@ -335,20 +336,27 @@ __global__ __launch_bounds__(KNN_NT) void k_knn(KnnArgs a) {
       for (int m = 0; m < MT; ++m)
 #pragma unroll
         for (int g = 0; g < 16; g += 4) {
-          float v[4], tg[4];
+          float v[4], tg[4], qg[4];
           bool pass[4];
           uint64_t any = 0;
           if constexpr (LEAN) {
             const float4 t4 = *reinterpret_cast<const float4*>(&sh_tf[w][m * 32 + rowof(g)]);
             tg[0] = t4.x; tg[1] = t4.y; tg[2] = t4.z; tg[3] = t4.w;
+            if constexpr (METRIC == RK_KNN_L2) {
+              const float4 q4 = *reinterpret_cast<const float4*>(&sh_qn[w][m * 32 + rowof(g)]);
+              qg[0] = q4.x; qg[1] = q4.y; qg[2] = q4.z; qg[3] = q4.w;
+            }
           } else {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) tg[e] = tf[m][g + e];
+            for (int e = 0; e < 4; ++e) {
+              tg[e] = tf[m][g + e];
+              qg[e] = qr[m][g + e];
+            }
           }
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             const int i = g + e;
-            v[e] = (METRIC == RK_KNN_IP) ? c[m][i] : knn_value(METRIC, c[m][i], qr[m][i], xn);
+            v[e] = (METRIC == RK_KNN_IP) ? c[m][i] : knn_value(METRIC, c[m][i], qg[e], xn);
             pass[e] = LARGEST ? !(v[e] < tg[e]) : !(v[e] > tg[e]);
             any |= __ballot(pass[e]);
           }
