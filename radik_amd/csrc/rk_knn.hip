@@ -198,11 +198,11 @@ __global__ __launch_bounds__(KNN_NT) __attribute__((amdgpu_waves_per_eu(MODE == 
 #pragma unroll
       for (int i = 0; i < 16; ++i) qr[m][i] = sh_qn[w][m * 32 + rowof(i)];
   }
-  // Per-row float thresholds (filter mode).  The inner-product filter keeps
-  // them in LDS (the four rows of a compare group are consecutive: one
-  // 16-byte load per group), which with 32-vector tiles and one prefetch set
-  // brings it to 126 registers, 4 waves per SIMD (~4 % faster than 3 in an
-  // A/B); L2, at 3 waves per SIMD either way, keeps them in registers.
+  // Per-row float thresholds and query norms (filter mode) stay in LDS: the
+  // four rows of a compare group are consecutive, so a group reads each with
+  // one 16-byte load.  With 32-vector tiles and one prefetch set this keeps
+  // the filter within 128 registers, 4 waves per SIMD at d <= 128 (measured
+  // 2 % faster than 3 for inner product, 8 % for L2).
   constexpr bool LEAN = (MODE == MODE_FILTER);
   __shared__ __attribute__((aligned(16))) float sh_tf[LEAN ? NW : 1][LEAN ? QW : 4];
   float tf[MT][16];
