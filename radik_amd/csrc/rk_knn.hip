@@ -663,6 +663,7 @@ hipError_t launch_knn(const KnnArgs& a0, hipStream_t s) {
   const int64_t per_xcd = (int64_t)a.qblocks * ((a.chunks + 7) / 8);
   const int64_t grid = per_xcd * 8;
   if (grid <= 0 || a.nq <= 0) return hipSuccess;
+  if (grid > 0x7fffffff) return hipErrorInvalidConfiguration;  // beyond one launch
   k_knn<DT, KS, MT, MODE, METRIC><<<(unsigned)grid, KNN_NT, 0, s>>>(a);
   return hipGetLastError();
 }
