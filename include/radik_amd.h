@@ -48,7 +48,8 @@ enum {
   RK_E_UNSUPPORTED = -3, /* sorted output with k above rk_max_k(), or dtype unknown */
   RK_E_ALIGN = -4,       /* value pointer not aligned to its element size */
   RK_E_HIP = -5,         /* a HIP runtime call failed */
-  RK_E_DEVICE = -6       /* device-side validation failed (rk_topk_status) */
+  RK_E_DEVICE = -6,      /* device-side validation failed (rk_topk_status) */
+  RK_E_IO = -7           /* batch / result file missing, unreadable or malformed */
 };
 
 /* Largest k for sorted output on the current device (16384 on MI355X: the
@@ -263,6 +264,39 @@ int rk_knn_topk(int dtype, int metric, const void* queries, const void* base, in
  * rk_knn_topk (the unfused path: scores to memory, then rk_topk_batched). */
 int rk_knn_scores(int dtype, int metric, const void* queries, const void* base, int32_t nq,
                   int64_t n, int32_t d, float* out, void* stream);
+
+/* ---- CSR batch and result files -----------------------------------------
+ * No counterpart in the reference, whose harness builds its rows in memory
+ * (radik/test/test_radik.cu:88-127): a little-endian file form of the
+ * batched entry's CSR layout (rk_topk_batched), for capturing a batch of
+ * requests and replaying it, and for storing its results.
+ *   batch file:  64-byte header (magic "RKBATCH1", dtype, flags, rows,
+ *                total), row_off int64[rows+1] (row_off[0] = 0, nondecreasing,
+ *                row_off[rows] = total), row_k int32[rows] padded to 8
+ *                bytes, values[total] (4 bytes for RK_F32, 2 for RK_F16 /
+ *                RK_BF16, as bits).
+ *   result file: 64-byte header (magic "RKRESLT1", dtype, 0, rows, outputs),
+ *                out_off int64[rows+1] (same rules, ending at outputs),
+ *                out_vals[outputs], out_idx int32[outputs].
+ * Buffers are host memory; no device work.  The *_info calls read the
+ * header (total = values of a batch, outputs of a result) so the caller can
+ * size the buffers of the matching *_read call, which validates the
+ * offsets.  Errors: RK_E_ARG (arguments), RK_E_UNSUPPORTED (dtype), RK_E_IO
+ * (the file). */
+typedef struct {
+  int32_t dtype;
+  uint32_t flags;  /* batch: the RK_* flags the batch was captured with */
+  int64_t rows;
+  int64_t total;
+} rk_file_info;
+int rk_batch_write(const char* path, int dtype, uint32_t flags, int32_t rows,
+                   const int64_t* row_off, const int32_t* row_k, const void* values);
+int rk_batch_info(const char* path, rk_file_info* info);
+int rk_batch_read(const char* path, int64_t* row_off, int32_t* row_k, void* values);
+int rk_result_write(const char* path, int dtype, int32_t rows, const int64_t* out_off,
+                    const void* out_vals, const int32_t* out_idx);
+int rk_result_info(const char* path, rk_file_info* info);
+int rk_result_read(const char* path, int64_t* out_off, void* out_vals, int32_t* out_idx);
 
 #ifdef __cplusplus
 }

@@ -47,6 +47,7 @@ _ERRORS = {
     0: "ok", -1: "bad argument", -2: "workspace missing or too small",
     -3: "unsupported dtype, or sorted output with k above rk_max_k()", -4: "misaligned pointer",
     -5: "HIP runtime error", -6: "device-side validation failed",
+    -7: "batch or result file missing, unreadable or malformed",
 }
 
 # Every symbol include/radik_amd.h and include/radik_amd_harness.h declare.
@@ -59,6 +60,8 @@ EXPORTED = (
     "rk_graph_destroy", "rk_probe_read", "rk_server_create", "rk_server_submit",
     "rk_server_wait", "rk_server_drain", "rk_server_replay", "rk_server_get_stats",
     "rk_server_destroy", "rk_knn_workspace_size", "rk_knn_topk", "rk_knn_scores",
+    "rk_batch_write", "rk_batch_info", "rk_batch_read", "rk_result_write", "rk_result_info",
+    "rk_result_read",
 )
 
 
@@ -103,6 +106,15 @@ def lib():
     if not os.path.exists(_LIB_PATH):
         raise RuntimeError(
             f"radik_amd: {_LIB_PATH} is missing; build it with `python -m radik_amd.build`")
+    # PyTorch-ROCm wheels carry their own HIP runtime, which torch loads by a
+    # path of its own.  Loaded first, this library would bind the system
+    # runtime, torch would then bring a second HIP/HSA runtime into the
+    # process, and this library's would find no device.  With torch imported
+    # first, the library binds torch's runtime (same soname): one runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(_LIB_PATH)
     c = ctypes
     vp, i32, i64, u32, sz = c.c_void_p, c.c_int32, c.c_int64, c.c_uint32, c.c_size_t
@@ -150,6 +162,16 @@ def lib():
                               vp]
     L.rk_knn_scores.argtypes = [c.c_int, c.c_int, vp, vp, i32, i64, i32, vp, vp]
     for name in ("rk_knn_workspace_size", "rk_knn_topk", "rk_knn_scores"):
+        getattr(L, name).restype = c.c_int
+    cp = c.c_char_p
+    L.rk_batch_write.argtypes = [cp, c.c_int, u32, i32, vp, vp, vp]
+    L.rk_batch_info.argtypes = [cp, vp]
+    L.rk_batch_read.argtypes = [cp, vp, vp, vp]
+    L.rk_result_write.argtypes = [cp, c.c_int, i32, vp, vp, vp]
+    L.rk_result_info.argtypes = [cp, vp]
+    L.rk_result_read.argtypes = [cp, vp, vp, vp]
+    for name in ("rk_batch_write", "rk_batch_info", "rk_batch_read", "rk_result_write",
+                 "rk_result_info", "rk_result_read"):
         getattr(L, name).restype = c.c_int
     L.rk_timing_enable.argtypes = [c.c_int]
     L.rk_timing_read.argtypes = [c.POINTER(c.c_double), c.POINTER(i32)]
@@ -464,6 +486,65 @@ def knn_scores(queries, base, *, metric: int = RK_KNN_IP, out=None, stream=None)
                              n, d, _ptr(out), _stream_ptr(stream))
     _check(rc, "rk_knn_scores")
     return out
+
+
+class _FileInfo(ctypes.Structure):
+    _fields_ = [("dtype", ctypes.c_int32), ("flags", ctypes.c_uint32), ("rows", ctypes.c_int64),
+                ("total", ctypes.c_int64)]
+
+
+_HOST_BITS = {RK_F32: np.uint32, RK_F16: np.uint16, RK_BF16: np.uint16}
+
+
+def _host(a, dtype=None):
+    a = np.ascontiguousarray(a if dtype is None else np.asarray(a, dtype=dtype))
+    return a, a.ctypes.data_as(ctypes.c_void_p)
+
+
+def save_batch(path: str, dtype: int, values_bits, row_off, row_k, flags: int = 0) -> None:
+    """Writes a CSR batch file (rk_batch_write).  ``values_bits``: host array of
+    the values' bit patterns (uint32 for RK_F32, uint16 for RK_F16 / RK_BF16);
+    ``row_off`` int64 [R+1] from 0; ``row_k`` int32 [R]; ``flags`` the RK_*
+    flags the batch is meant for."""
+    v, vp = _host(values_bits, _HOST_BITS[dtype])
+    o, op = _host(row_off, np.int64)
+    k, kp = _host(row_k, np.int32)
+    _check(lib().rk_batch_write(path.encode(), dtype, flags, len(k), op, kp, vp), "rk_batch_write")
+
+
+def load_batch(path: str):
+    """Reads a CSR batch file: (dtype, flags, values bits, row_off, row_k) as
+    host numpy arrays."""
+    info = _FileInfo()
+    _check(lib().rk_batch_info(path.encode(), ctypes.byref(info)), "rk_batch_info")
+    v = np.empty(info.total, dtype=_HOST_BITS[info.dtype])
+    o = np.empty(info.rows + 1, dtype=np.int64)
+    k = np.empty(info.rows, dtype=np.int32)
+    _check(lib().rk_batch_read(path.encode(), o.ctypes.data_as(ctypes.c_void_p),
+                               k.ctypes.data_as(ctypes.c_void_p),
+                               v.ctypes.data_as(ctypes.c_void_p)), "rk_batch_read")
+    return info.dtype, info.flags, v, o, k
+
+
+def save_results(path: str, dtype: int, out_off, out_vals_bits, out_idx) -> None:
+    """Writes a result file (rk_result_write) from host arrays."""
+    o, op = _host(out_off, np.int64)
+    v, vp = _host(out_vals_bits, _HOST_BITS[dtype])
+    i, ip = _host(out_idx, np.int32)
+    _check(lib().rk_result_write(path.encode(), dtype, len(o) - 1, op, vp, ip), "rk_result_write")
+
+
+def load_results(path: str):
+    """Reads a result file: (dtype, out_off, out_vals bits, out_idx)."""
+    info = _FileInfo()
+    _check(lib().rk_result_info(path.encode(), ctypes.byref(info)), "rk_result_info")
+    o = np.empty(info.rows + 1, dtype=np.int64)
+    v = np.empty(info.total, dtype=_HOST_BITS[info.dtype])
+    i = np.empty(info.total, dtype=np.int32)
+    _check(lib().rk_result_read(path.encode(), o.ctypes.data_as(ctypes.c_void_p),
+                                v.ctypes.data_as(ctypes.c_void_p),
+                                i.ctypes.data_as(ctypes.c_void_p)), "rk_result_read")
+    return info.dtype, o, v, i
 
 
 def probe_read(t, blocks_per_cu: int = 0, stream=None) -> None:

@@ -26,6 +26,7 @@ SOURCES = [
     "rk_tune.cpp",
     "rk_serve.cpp",
     "rk_knn.hip",
+    "rk_io.cpp",
 ]
 
 COMMON = [

@@ -361,6 +361,7 @@ const char* rk_strerror(int code) {
     case RK_E_ALIGN: return "pointer not aligned to its element size (workspace: 256 bytes)";
     case RK_E_HIP: return "HIP runtime error";
     case RK_E_DEVICE: return "device-side validation failed";
+    case RK_E_IO: return "batch or result file missing, unreadable or malformed";
     default: return "unknown error";
   }
 }

@@ -1,0 +1,119 @@
+"""CSR batch and result files (rk_batch_* / rk_result_*): host-only, so
+tested on the CPU: round trips for every dtype (ragged and empty rows, an
+empty batch), the header fields, and rejection of malformed files and
+arguments."""
+
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+import radik_amd as rk
+
+DTYPES = [(rk.RK_F32, np.uint32), (rk.RK_F16, np.uint16), (rk.RK_BF16, np.uint16)]
+
+
+def _batch(seed, dt_bits, rows=7):
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 50, rows)
+    lens[2] = 0  # an empty row
+    off = np.zeros(rows + 1, np.int64)
+    off[1:] = np.cumsum(lens)
+    ks = rng.integers(0, 20, rows).astype(np.int32)
+    bits = rng.integers(0, np.iinfo(dt_bits).max, int(off[-1]), dtype=np.uint64).astype(dt_bits)
+    return bits, off, ks
+
+
+@pytest.mark.parametrize("dtype,bits_t", DTYPES)
+def test_batch_round_trip(tmp_path, dtype, bits_t):
+    bits, off, ks = _batch(dtype + 3, bits_t)
+    path = str(tmp_path / "b.rkb")
+    flags = rk.RK_LARGEST | rk.RK_SORTED
+    rk.save_batch(path, dtype, bits, off, ks, flags=flags)
+    dt, fl, v, o, k = rk.load_batch(path)
+    assert dt == dtype and fl == flags
+    assert v.dtype == bits_t
+    np.testing.assert_array_equal(v, bits)
+    np.testing.assert_array_equal(o, off)
+    np.testing.assert_array_equal(k, ks)
+    # header 64 + offsets + k padded to 8 + values
+    kb = 4 * len(ks)
+    assert os.path.getsize(path) == 64 + 8 * len(off) + kb + (8 - kb % 8) % 8 + bits.nbytes
+
+
+def test_empty_batch(tmp_path):
+    path = str(tmp_path / "e.rkb")
+    rk.save_batch(path, rk.RK_F32, np.zeros(0, np.uint32), np.zeros(1, np.int64),
+                  np.zeros(0, np.int32))
+    dt, fl, v, o, k = rk.load_batch(path)
+    assert dt == rk.RK_F32 and len(v) == 0 and list(o) == [0] and len(k) == 0
+
+
+@pytest.mark.parametrize("dtype,bits_t", DTYPES)
+def test_result_round_trip(tmp_path, dtype, bits_t):
+    rng = np.random.default_rng(dtype)
+    m = rng.integers(0, 9, 5)
+    off = np.zeros(6, np.int64)
+    off[1:] = np.cumsum(m)
+    vals = rng.integers(0, 1 << 16, int(off[-1])).astype(bits_t)
+    idx = rng.integers(-1, 1000, int(off[-1])).astype(np.int32)
+    path = str(tmp_path / "r.rkr")
+    rk.save_results(path, dtype, off, vals, idx)
+    dt, o, v, i = rk.load_results(path)
+    assert dt == dtype
+    np.testing.assert_array_equal(o, off)
+    np.testing.assert_array_equal(v, vals)
+    np.testing.assert_array_equal(i, idx)
+
+
+def _code(fn):
+    with pytest.raises(rk.RadikError) as e:
+        fn()
+    return e.value.code
+
+
+def test_malformed_files_rejected(tmp_path):
+    bits, off, ks = _batch(1, np.uint32)
+    good = str(tmp_path / "g.rkb")
+    rk.save_batch(good, rk.RK_F32, bits, off, ks)
+    raw = open(good, "rb").read()
+    cases = {
+        "missing": None,
+        "magic": b"XKBATCH1" + raw[8:],
+        "truncated": raw[:-3],
+        "trailing": raw + b"\0",
+        "dtype": raw[:8] + (9).to_bytes(4, "little") + raw[12:],
+    }
+    # offsets that decrease: swap row_off[1] and row_off[3] (both nonzero)
+    o = np.frombuffer(raw[64:64 + 8 * len(off)], np.int64).copy()
+    o[1], o[3] = o[3], o[1]
+    cases["offsets"] = raw[:64] + o.tobytes() + raw[64 + 8 * len(off):]
+    for name, data in cases.items():
+        p = str(tmp_path / f"{name}.rkb")
+        if data is not None:
+            open(p, "wb").write(data)
+        assert _code(lambda: rk.load_batch(p)) == -7, name
+    # a result file is not a batch file, and vice versa
+    assert _code(lambda: rk.load_results(good)) == -7
+
+
+def test_bad_arguments(tmp_path):
+    p = str(tmp_path / "x.rkb")
+    bits = np.zeros(4, np.uint32)
+    ks = np.ones(2, np.int32)
+    # offsets not starting at 0 / decreasing
+    assert _code(lambda: rk.save_batch(p, rk.RK_F32, bits, [1, 2, 4], ks)) == -1
+    assert _code(lambda: rk.save_batch(p, rk.RK_F32, bits, [0, 3, 2], ks)) == -1
+    # negative k
+    assert _code(lambda: rk.save_batch(p, rk.RK_F32, bits, [0, 2, 4], [1, -1])) == -1
+    # unknown dtype through the C ABI
+    L = rk.lib()
+    off = np.array([0, 4], np.int64)
+    k1 = np.ones(1, np.int32)
+    rc = L.rk_batch_write(p.encode(), 7, 0, 1, off.ctypes.data_as(rk.ctypes.c_void_p),
+                          k1.ctypes.data_as(rk.ctypes.c_void_p),
+                          bits.ctypes.data_as(rk.ctypes.c_void_p))
+    assert rc == -3
+    assert L.rk_batch_info(None, None) == -1
