@@ -33,6 +33,7 @@ if ROOT not in sys.path:
 
 METRIC = "top-k input keys/s + % HBM roofline (batched vs plain); serving p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
+REPS = 20  # individually timed graph replays (median and spread)
 
 
 def log(msg: str) -> None:
@@ -197,6 +198,15 @@ def main() -> int:
     g1.record(gstream)
     gstream.synchronize()
     graph_ms = g0.elapsed_time(g1) / args.steps
+    # Per-replay times as well (a separate pass, so the bracketed steps above
+    # carry no per-step events): the median and spread of 20 replays.
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(REPS + 1)]
+    ev[0].record(gstream)
+    for i in range(REPS):
+        graph.launch(gstream)
+        ev[i + 1].record(gstream)
+    gstream.synchronize()
+    reps_ms = np.array([ev[i].elapsed_time(ev[i + 1]) for i in range(REPS)])
     graph.close()
     st = rk.status(ws)
     if st[0] != 0:
@@ -246,6 +256,9 @@ def main() -> int:
         "pipeline": {"algorithmic_bytes": alg_bytes, "achieved_gbs": pipeline_gbs,
                      "roofline_frac": pipeline_gbs / HBM_PEAK_GBS,
                      "eager_ms_per_step": eager_ms, "graph_ms_per_step": graph_ms,
+                     "graph_replay_ms": {"reps": REPS, "median": float(np.median(reps_ms)),
+                                         "min": float(reps_ms.min()),
+                                         "max": float(reps_ms.max())},
                      "tunables": tuned or "defaults",
                      "fallback_rows": st[1], "ordered_rows": st[2]},
     }
