@@ -482,21 +482,51 @@ RK_DEV void bitonic_sort_e(uint64_t* a, uint32_t P2) {
   __syncthreads();
 }
 
-// Plain LDS bitonic sort (every stage through LDS), for the largest sizes
-// where keeping E keys per thread in registers would spill.
+// Compare-exchange of x (lower index) and y in a bitonic stage.
+RK_DEV void bitonic_cas(uint64_t& x, uint64_t& y, bool desc) {
+  if (desc ? (x < y) : (x > y)) {
+    const uint64_t t = x;
+    x = y;
+    y = t;
+  }
+}
+
+// LDS bitonic sort for the largest sizes, where keeping E keys per thread in
+// registers would spill.  Stages go through LDS two at a time: a thread
+// loads the four keys b, b+h, b+2h, b+3h (b with bits h and 2h clear) and
+// applies the stage at distance 2h and then the one at h in registers, so
+// every key is read and written once per two stages, with one barrier.  A
+// size with an odd number of stages ends with a single stage at distance 1.
 template <int NT>
 RK_DEV void bitonic_sort_lds(uint64_t* a, uint32_t P2) {
   for (uint32_t size = 2; size <= P2; size <<= 1) {
-    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+    uint32_t stride = size >> 1;
+    for (; stride >= 2; stride >>= 2) {
+      const uint32_t h = stride >> 1;
+      const int lh = __builtin_ctz(h);
+      for (uint32_t g = threadIdx.x; g < (P2 >> 2); g += NT) {
+        const uint32_t b = (g & (h - 1)) | ((g >> lh) << (lh + 2));
+        const bool desc = (b & size) == 0;  // the four keys share it: h, 2h < size
+        uint64_t x0 = a[b], x1 = a[b + h], x2 = a[b + 2 * h], x3 = a[b + 3 * h];
+        bitonic_cas(x0, x2, desc);
+        bitonic_cas(x1, x3, desc);
+        bitonic_cas(x0, x1, desc);
+        bitonic_cas(x2, x3, desc);
+        a[b] = x0;
+        a[b + h] = x1;
+        a[b + 2 * h] = x2;
+        a[b + 3 * h] = x3;
+      }
+      __syncthreads();
+    }
+    if (stride == 1) {
       for (uint32_t i = threadIdx.x; i < (P2 >> 1); i += NT) {
-        const uint32_t lo = 2 * i - (i & (stride - 1));
-        const uint32_t hi = lo + stride;
+        const uint32_t lo = 2 * i;
         const bool desc = (lo & size) == 0;
-        const uint64_t x = a[lo], y = a[hi];
-        if (desc ? (x < y) : (x > y)) {
-          a[lo] = y;
-          a[hi] = x;
-        }
+        uint64_t x = a[lo], y = a[lo + 1];
+        bitonic_cas(x, y, desc);
+        a[lo] = x;
+        a[lo + 1] = y;
       }
       __syncthreads();
     }
