@@ -117,3 +117,35 @@ def test_bad_arguments(tmp_path):
                           bits.ctypes.data_as(rk.ctypes.c_void_p))
     assert rc == -3
     assert L.rk_batch_info(None, None) == -1
+
+
+def test_read_refuses_a_file_larger_than_the_buffers(tmp_path):
+    """*_read takes the sizes the caller allocated for (from *_info): a file
+    replaced by a larger one in between is refused before anything is
+    written, for both file kinds."""
+    L = rk.lib()
+    vp = rk.ctypes.c_void_p
+    small, big = str(tmp_path / "s.rkb"), str(tmp_path / "b.rkb")
+    rk.save_batch(small, rk.RK_F32, np.arange(4, dtype=np.uint32), [0, 4], [2])
+    rk.save_batch(big, rk.RK_F32, np.arange(40, dtype=np.uint32), [0, 10, 40], [2, 3])
+    off = np.full(2, -5, np.int64)  # sized for the small file: 1 row, 4 values
+    ks = np.full(1, -5, np.int32)
+    v = np.full(4, 7, np.uint32)
+    args = (off.ctypes.data_as(vp), ks.ctypes.data_as(vp), v.ctypes.data_as(vp))
+    assert L.rk_batch_read(big.encode(), 1, 4, *args) == -7
+    assert list(off) == [-5, -5] and list(ks) == [-5] and list(v) == [7] * 4
+    assert L.rk_batch_read(small.encode(), 1, 4, *args) == 0
+    assert list(off) == [0, 4] and list(ks) == [2] and list(v) == [0, 1, 2, 3]
+    assert L.rk_batch_read(small.encode(), -1, 4, *args) == -1
+
+    rs, rb = str(tmp_path / "s.rkr"), str(tmp_path / "b.rkr")
+    rk.save_results(rs, rk.RK_F32, [0, 2], np.ones(2, np.uint32), np.zeros(2, np.int32))
+    rk.save_results(rb, rk.RK_F32, [0, 2, 9], np.ones(9, np.uint32), np.zeros(9, np.int32))
+    ro = np.full(2, -5, np.int64)
+    rv = np.full(2, 7, np.uint32)
+    ri = np.full(2, 7, np.int32)
+    rargs = (ro.ctypes.data_as(vp), rv.ctypes.data_as(vp), ri.ctypes.data_as(vp))
+    assert L.rk_result_read(rb.encode(), 1, 2, *rargs) == -7
+    assert list(ro) == [-5, -5] and list(rv) == [7, 7]
+    assert L.rk_result_read(rs.encode(), 1, 2, *rargs) == 0
+    assert list(ro) == [0, 2] and list(rv) == [1, 1] and list(ri) == [0, 0]
