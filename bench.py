@@ -60,6 +60,7 @@ def make_workload(name: str):
     table = {
         "c4_f32": lambda: W.c4("f32"),
         "c4_bf16": lambda: W.c4("bf16"),
+        "c4_f16": lambda: W.c4("f16"),
         "c2": lambda: W.c2(),
         "c1": lambda: W.c1(),
         "c3": lambda: W.c3(),
@@ -73,7 +74,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c4_f32",
-                    choices=["c4_f32", "c4_bf16", "c2", "c1", "c3", "knn"],
+                    choices=["c4_f32", "c4_bf16", "c4_f16", "c2", "c1", "c3", "knn"],
                     help="knn: the fused k-nearest-neighbour select against its unfused "
                          "form (prints its own JSON object)")
     ap.add_argument("--knn-shape", default="1024,1048576,128,100",

@@ -9,7 +9,7 @@ Shapes follow the five configurations the top-k path is measured on:
   duplicates, rows 32-63 Zipf(s = 1.1) ranks over 2^20 values mapped to
   rank / 2^20 (seed 3);
 * ``c4``: 2048 rows, len = floor(2^U(10, 24)), k = floor(2^U(0, log2 min(len,
-  4096))), N(0,1), fp32 or bf16 rounded from the same fp32 (seed 4);
+  4096))), N(0,1), fp32, or bf16 / f16 rounded from the same fp32 (seed 4);
 * ``c5``: serving requests, len 2^U(12, 22), k 2^U(0, 12), a mixture of
   uniform / normal / Zipf / near-constant rows, Poisson arrivals (seed 5).
 

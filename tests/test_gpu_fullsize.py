@@ -1,7 +1,7 @@
 """The benchmark workloads at their full sizes.
 
 C3 (64 x 2^22, skewed / duplicate-heavy, k = 16384) is checked against the
-oracle bit-exactly.  C4 (2048 uneven rows, 3.39 G keys, fp32 and bf16) is too
+oracle bit-exactly.  C4 (2048 uneven rows, 3.39 G keys, fp32, bf16 and f16) is too
 large for the oracle in a test, so every row is checked on the device with
 size-independent properties: the selected values equal torch.topk's values
 (the multiset of the k largest), every index points at its value, and the
@@ -37,7 +37,7 @@ def test_c3_full_size_against_oracle():
         np.testing.assert_array_equal(from_device_bits(ov, "f32"), eb)
 
 
-@pytest.mark.parametrize("dt", ["f32", "bf16"])
+@pytest.mark.parametrize("dt", ["f32", "bf16", "f16"])
 def test_c4_full_size_properties(dt):
     import torch
 
