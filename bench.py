@@ -212,6 +212,10 @@ def main() -> int:
     st = rk.status(ws)
     if st[0] != 0:
         raise RuntimeError(f"device status after graph replays {st}")
+    # The timed calls' output (the last replay's), kept for the bit-exact
+    # check against the oracle in the CPU-baseline leg.
+    timed_out = (out_v.view(torch.int32 if es == 4 else torch.int16).cpu().numpy(),
+                 out_i.cpu().numpy())
     log(f"graph replay: {graph_ms:.3f} ms/step ({w.total / (graph_ms / 1e3) / 1e9:.2f} Gkeys/s)")
     eager_ms = ms_step
     ms_step = min(ms_step, graph_ms)
@@ -327,13 +331,26 @@ def main() -> int:
         iview = vals.view(torch.int32 if es == 4 else torch.int16)
         for j, r in enumerate(rows):
             host[s_off[j]:s_off[j + 1]] = iview[off[r]:off[r + 1]].cpu().numpy().view(host.dtype)
-        nth = min(16, orc.hardware_threads())
+        nth = orc.usable_threads()
         c0 = time.perf_counter()
-        orc.topk(host, s_off, w.ks[rows], dt=dt, flags=orc.LARGEST | orc.SORTED, nthreads=nth)
+        eb, ei = orc.topk(host, s_off, w.ks[rows], dt=dt, flags=orc.LARGEST | orc.SORTED,
+                          nthreads=nth)
         c_s = time.perf_counter() - c0
+        # The timed GPU output of the sampled rows must equal the oracle's.
+        sel = np.concatenate([np.arange(oo[r], oo[r + 1]) for r in rows])
+        gv = timed_out[0][sel].view(eb.dtype)
+        exact = bool(np.array_equal(gv, eb) and np.array_equal(timed_out[1][sel], ei))
+        result["bit_exact"] = exact
+        result["bit_exact_checked"] = {"rows": int(len(rows)), "outputs": int(len(sel)),
+                                       "against": "oracle (CPU restatement), values and indices"}
+        if not exact:
+            log("timed output differs from the oracle")
+            print(json.dumps(result), flush=True)
+            return 1
         result["cpu_baseline"] = {
             "value": float(s_off[-1]) / c_s, "unit": "keys/s", "cores": nth, "kind": "port",
             "cpu_model": cpu_model(), "host_threads": orc.hardware_threads(),
+            "threads_rule": "every CPU of this process's affinity mask within the cgroup quota",
             "sample": ("the whole workload" if stride == 1 else
                        f"every {stride}th row of the workload")
                       + f" ({len(rows)} rows, {int(s_off[-1])} keys), std::nth_element + "

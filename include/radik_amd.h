@@ -279,12 +279,13 @@ int rk_knn_scores(int dtype, int metric, const void* queries, const void* base, 
  *                out_off int64[rows+1] (same rules, ending at outputs),
  *                out_vals[outputs], out_idx int32[outputs].
  * Buffers are host memory; no device work.  The *_info calls read the
- * header (total = values of a batch, outputs of a result) so the caller can
- * size the buffers of the matching *_read call.  That call takes those
- * sizes back (max_rows, max_total: what the buffers hold) and rejects a
- * file that has grown past them since (RK_E_IO) before writing anything;
- * it validates the offsets and the file's exact length.  Errors: RK_E_ARG
- * (arguments), RK_E_UNSUPPORTED (dtype), RK_E_IO (the file). */
+ * header (dtype; total = values of a batch, outputs of a result) so the
+ * caller can size the buffers of the matching *_read call.  That call takes
+ * the dtype and sizes back (max_rows, max_total: what the buffers hold) and
+ * rejects a file that has since changed its element type or grown past them
+ * (RK_E_IO) before writing anything; it validates the offsets and the
+ * file's exact length.  Errors: RK_E_ARG (arguments), RK_E_UNSUPPORTED
+ * (dtype), RK_E_IO (the file). */
 typedef struct {
   int32_t dtype;
   uint32_t flags;  /* batch: the RK_* flags the batch was captured with */
@@ -294,13 +295,13 @@ typedef struct {
 int rk_batch_write(const char* path, int dtype, uint32_t flags, int32_t rows,
                    const int64_t* row_off, const int32_t* row_k, const void* values);
 int rk_batch_info(const char* path, rk_file_info* info);
-int rk_batch_read(const char* path, int64_t max_rows, int64_t max_total, int64_t* row_off,
-                  int32_t* row_k, void* values);
+int rk_batch_read(const char* path, int dtype, int64_t max_rows, int64_t max_total,
+                  int64_t* row_off, int32_t* row_k, void* values);
 int rk_result_write(const char* path, int dtype, int32_t rows, const int64_t* out_off,
                     const void* out_vals, const int32_t* out_idx);
 int rk_result_info(const char* path, rk_file_info* info);
-int rk_result_read(const char* path, int64_t max_rows, int64_t max_total, int64_t* out_off,
-                   void* out_vals, int32_t* out_idx);
+int rk_result_read(const char* path, int dtype, int64_t max_rows, int64_t max_total,
+                   int64_t* out_off, void* out_vals, int32_t* out_idx);
 
 #ifdef __cplusplus
 }

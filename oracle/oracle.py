@@ -65,6 +65,26 @@ def hardware_threads() -> int:
     return int(lib().orc_hardware_threads())
 
 
+def usable_threads() -> int:
+    """Threads the multithreaded oracle may use here: the CPUs in this
+    process's affinity mask, bounded by the cgroup CPU quota when one is set
+    (a container's share of a larger host), else all hardware threads."""
+    n = hardware_threads()
+    try:
+        n = min(n, len(os.sched_getaffinity(0)))
+    except (AttributeError, OSError):
+        pass
+    for path in ("/sys/fs/cgroup/cpu.max",):
+        try:
+            with open(path) as f:
+                quota, period = f.read().split()[:2]
+            if quota != "max":
+                n = min(n, max(1, -(-int(quota) // int(period))))
+        except (OSError, ValueError):
+            pass
+    return max(1, n)
+
+
 def _ptr(a):
     return None if a is None else a.ctypes.data
 

@@ -166,10 +166,10 @@ def lib():
     cp = c.c_char_p
     L.rk_batch_write.argtypes = [cp, c.c_int, u32, i32, vp, vp, vp]
     L.rk_batch_info.argtypes = [cp, vp]
-    L.rk_batch_read.argtypes = [cp, i64, i64, vp, vp, vp]
+    L.rk_batch_read.argtypes = [cp, i32, i64, i64, vp, vp, vp]
     L.rk_result_write.argtypes = [cp, c.c_int, i32, vp, vp, vp]
     L.rk_result_info.argtypes = [cp, vp]
-    L.rk_result_read.argtypes = [cp, i64, i64, vp, vp, vp]
+    L.rk_result_read.argtypes = [cp, i32, i64, i64, vp, vp, vp]
     for name in ("rk_batch_write", "rk_batch_info", "rk_batch_read", "rk_result_write",
                  "rk_result_info", "rk_result_read"):
         getattr(L, name).restype = c.c_int
@@ -520,7 +520,7 @@ def load_batch(path: str):
     v = np.empty(info.total, dtype=_HOST_BITS[info.dtype])
     o = np.empty(info.rows + 1, dtype=np.int64)
     k = np.empty(info.rows, dtype=np.int32)
-    _check(lib().rk_batch_read(path.encode(), info.rows, info.total,
+    _check(lib().rk_batch_read(path.encode(), info.dtype, info.rows, info.total,
                                o.ctypes.data_as(ctypes.c_void_p),
                                k.ctypes.data_as(ctypes.c_void_p),
                                v.ctypes.data_as(ctypes.c_void_p)), "rk_batch_read")
@@ -542,7 +542,7 @@ def load_results(path: str):
     o = np.empty(info.rows + 1, dtype=np.int64)
     v = np.empty(info.total, dtype=_HOST_BITS[info.dtype])
     i = np.empty(info.total, dtype=np.int32)
-    _check(lib().rk_result_read(path.encode(), info.rows, info.total,
+    _check(lib().rk_result_read(path.encode(), info.dtype, info.rows, info.total,
                                 o.ctypes.data_as(ctypes.c_void_p),
                                 v.ctypes.data_as(ctypes.c_void_p),
                                 i.ctypes.data_as(ctypes.c_void_p)), "rk_result_read")

@@ -87,20 +87,54 @@ RK_DEV bool bits_is_nan(uint32_t u) {
   else return (u & 0x7fffu) > 0x7f80u;
 }
 
-// Float threshold for a key threshold T > 0, such that for every non-NaN x:
-//   sel_key(x) >= T  implies  (LARGEST ? x >= tf : x <= tf).
-// (The converse can fail only for -0/+0, which only adds candidates.)
+// Signed-order image of a bit pattern: as a signed integer of the element's
+// width, ord(u) orders exactly as the flip does (ord(u) = flip(u) with the top
+// bit toggled): -NaN < -inf < ... < -0 < +0 < ... < +inf < +NaN.  Integer
+// only, so it does not depend on the float unit's NaN or denormal handling
+// (a signalling NaN stays an ordinary bit pattern).  ord32 maps one f32
+// word; ord16x2 maps the two 16-bit elements of a word independently.
+RK_DEV int32_t ord32(uint32_t u) {
+  return (int32_t)(u ^ ((uint32_t)((int32_t)u >> 31) >> 1));
+}
+RK_DEV uint32_t ord16x2(uint32_t w) {
+  return w ^ (((w >> 15) & 0x00010001u) * 0x7fffu);
+}
+
+// Threshold of the vector prefilter in ord space for a key threshold T > 0:
+// for every non-NaN element u,
+//   sel_key(u) >= T  <=>  (LARGEST ? ord(u) >= t : ord(u) <= t).
+// A NaN may pass the ord compare (its sel_key is 0 < T), so the prefilter
+// admits a superset; the exact per-element test is sel_key(u) >= T.
 template <int DT, bool LARGEST>
-RK_DEV float key_threshold_float(uint32_t T) {
-  constexpr uint32_t HALF = 1u << (Traits<DT>::KEY_BITS - 1);
-  const uint32_t b = key_to_bits<DT, LARGEST>(T);
-  if (bits_is_nan<DT>(b)) {
-    // T lies in the part of key space no non-NaN value occupies.
-    const bool low = T < HALF;
-    const float inf = __builtin_inff();
-    return low ? (LARGEST ? -inf : inf) : (LARGEST ? inf : -inf);
+RK_DEV int32_t ord_threshold(uint32_t T) {
+  constexpr int KB = Traits<DT>::KEY_BITS;
+  constexpr uint32_t KM = Traits<DT>::KEY_MASK, TOP = 1u << (KB - 1);
+  const uint32_t f = (LARGEST ? T : ~T) & KM;  // the flip at the boundary
+  const uint32_t o = f ^ TOP;
+  if constexpr (KB == 32) return (int32_t)o;
+  else return (int32_t)(int16_t)(uint16_t)o;
+}
+
+// Best element of a 16-byte vector in ord space (max for largest-k, min for
+// smallest-k): the vector can hold a key >= T only if this passes the
+// ord_threshold compare, one integer compare per vector.
+typedef short rk_s16x2 __attribute__((ext_vector_type(2)));
+template <int DT, bool LARGEST>
+RK_DEV int32_t vec_best_ord(const uint4& x) {
+  if constexpr (DT == F32) {
+    const int32_t a = ord32(x.x), b = ord32(x.y), c = ord32(x.z), d = ord32(x.w);
+    return LARGEST ? max(max(a, b), max(c, d)) : min(min(a, b), min(c, d));
+  } else {
+    auto h = [](uint32_t w) { return __builtin_bit_cast(rk_s16x2, ord16x2(w)); };
+    const rk_s16x2 a = h(x.x), b = h(x.y), c = h(x.z), d = h(x.w);
+    rk_s16x2 m;
+    if constexpr (LARGEST)
+      m = __builtin_elementwise_max(__builtin_elementwise_max(a, b), __builtin_elementwise_max(c, d));
+    else
+      m = __builtin_elementwise_min(__builtin_elementwise_min(a, b), __builtin_elementwise_min(c, d));
+    const int32_t lo = m.x, hi = m.y;
+    return LARGEST ? max(lo, hi) : min(lo, hi);
   }
-  return bits_to_float<DT>(b);
 }
 
 // Scalar loads through explicit address spaces.  Generic pointers may alias
@@ -161,32 +195,6 @@ RK_DEV float bload4f(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
 // A byte offset beyond every range a descriptor here describes (loads at it
 // return zeros).
 constexpr uint32_t RSRC_OOB = 0xfffffff0u;
-
-// Best value of a 16-byte vector in the selection direction (max for
-// largest-k, min for smallest-k), as fp32.  max/min ignore a NaN operand, so
-// a vector is "all NaN or below the threshold" exactly when this value fails
-// the threshold compare: one compare per vector instead of one per element.
-template <int DT, bool LARGEST>
-RK_DEV float vec_best(const uint4& x) {
-  auto pick = [](float a, float b) { return LARGEST ? fmaxf(a, b) : fminf(a, b); };
-  if constexpr (DT == F32) {
-    return pick(pick(__uint_as_float(x.x), __uint_as_float(x.y)),
-                pick(__uint_as_float(x.z), __uint_as_float(x.w)));
-  } else if constexpr (DT == BF16) {
-    const float a = pick(__uint_as_float(x.x << 16), __uint_as_float(x.x & 0xffff0000u));
-    const float b = pick(__uint_as_float(x.y << 16), __uint_as_float(x.y & 0xffff0000u));
-    const float c = pick(__uint_as_float(x.z << 16), __uint_as_float(x.z & 0xffff0000u));
-    const float d = pick(__uint_as_float(x.w << 16), __uint_as_float(x.w & 0xffff0000u));
-    return pick(pick(a, b), pick(c, d));
-  } else {
-    auto h = [](uint32_t u) { return (float)__builtin_bit_cast(_Float16, (uint16_t)u); };
-    const float a = pick(h(x.x & 0xffffu), h(x.x >> 16));
-    const float b = pick(h(x.y & 0xffffu), h(x.y >> 16));
-    const float c = pick(h(x.z & 0xffffu), h(x.z >> 16));
-    const float d = pick(h(x.w & 0xffffu), h(x.w >> 16));
-    return pick(pick(a, b), pick(c, d));
-  }
-}
 
 // Unpacked 16-byte vector of PACK elements (bit patterns, low element first).
 template <int DT>

@@ -111,14 +111,19 @@ extern "C" int rk_batch_info(const char* path, rk_file_info* info) {
   return info_of(path, BATCH_MAGIC, info);
 }
 
-extern "C" int rk_batch_read(const char* path, int64_t max_rows, int64_t max_total,
+extern "C" int rk_batch_read(const char* path, int dtype, int64_t max_rows, int64_t max_total,
                              int64_t* row_off, int32_t* row_k, void* values) {
   if (!path || !row_off || max_rows < 0 || max_total < 0) return RK_E_ARG;
+  if (elem_bytes(dtype) == 0) return RK_E_UNSUPPORTED;
   File f(path, "rb");
   FileHeader h;
   int rc = read_header(f, BATCH_MAGIC, &h);
   if (rc != RK_OK) return rc;
-  if (h.rows > max_rows || h.total > max_total) return RK_E_IO;  // larger than the buffers
+  // The buffers were sized for `dtype`, max_rows and max_total (from
+  // rk_batch_info): a file rewritten since with another element type or more
+  // data is refused before anything is written.
+  if (h.dtype != dtype) return RK_E_IO;
+  if (h.rows > max_rows || h.total > max_total) return RK_E_IO;
   if ((h.rows && !row_k) || (h.total && !values)) return RK_E_ARG;
   const size_t kbytes = (size_t)h.rows * 4;
   uint8_t pad[8];
@@ -155,13 +160,15 @@ extern "C" int rk_result_info(const char* path, rk_file_info* info) {
   return info_of(path, RESULT_MAGIC, info);
 }
 
-extern "C" int rk_result_read(const char* path, int64_t max_rows, int64_t max_total,
+extern "C" int rk_result_read(const char* path, int dtype, int64_t max_rows, int64_t max_total,
                               int64_t* out_off, void* out_vals, int32_t* out_idx) {
   if (!path || !out_off || max_rows < 0 || max_total < 0) return RK_E_ARG;
+  if (elem_bytes(dtype) == 0) return RK_E_UNSUPPORTED;
   File f(path, "rb");
   FileHeader h;
   int rc = read_header(f, RESULT_MAGIC, &h);
   if (rc != RK_OK) return rc;
+  if (h.dtype != dtype) return RK_E_IO;  // buffers sized for another element type
   if (h.rows > max_rows || h.total > max_total) return RK_E_IO;  // larger than the buffers
   if (h.total && (!out_vals || !out_idx)) return RK_E_ARG;
   if (!f.read(out_off, ((size_t)h.rows + 1) * 8)) return RK_E_IO;

@@ -534,8 +534,9 @@ __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
 // filter: the streaming pass.  Every element of every PH_FILTER row is read
 // once with 16-byte loads; keys >= T are appended (as composite keys) to the
 // row's candidate slots, staged in LDS and flushed with one global atomic
-// per segment.  The compare is done on the float value (one VALU op per
-// element); it admits every key >= T plus possibly -0/+0 neighbours.
+// per segment.  Each 16-byte vector is first tested as a whole in integer
+// ord space (one compare per vector); only vectors that pass are tested per
+// element on the selection key, so exactly the keys >= T are admitted.
 // ---------------------------------------------------------------------------
 // Candidate staging of the streaming filter.  Each wave owns a private LDS
 // stage of WSTAGE composite keys; only that wave writes or reads it (LDS
@@ -619,7 +620,7 @@ void k_filter(KParams p) {
     const uint32_t T = st.T;
     const uint32_t cap = st.cap;
     const bool all = (T == 0);
-    const float tf = all ? 0.0f : key_threshold_float<DT, LARGEST>(T);
+    const int32_t to = all ? 0 : ord_threshold<DT, LARGEST>(T);
     const UT* rb = static_cast<const UT*>(d.base);
     // 16-byte vectors are counted from a0, the row's first element on a
     // 128-byte boundary (an L2 line): a wave's 1 KB load then covers 8
@@ -641,11 +642,7 @@ void k_filter(KParams p) {
     const bool last_seg = (int64_t)tb * p.tune.tile >= d.len;
     const int64_t tail0 = a0 + nv_row * P;
 
-    auto pred = [&](uint32_t u) -> bool {
-      if (all) return true;
-      const float x = bits_to_float<DT>(u);
-      return LARGEST ? (x >= tf) : (x <= tf);
-    };
+    auto pred = [&](uint32_t u) -> bool { return all || sel_key<DT, LARGEST>(u) >= T; };
     AppendCtx ac;
     ac.stage = stage[wave_base >> 6];
     ac.cand = p.cand + cbase;
@@ -698,8 +695,8 @@ void k_filter(KParams p) {
           uint32_t hb = 0;  // bit u: this lane's vector u holds a candidate
 #pragma unroll
           for (int u = 0; u < U; ++u) {
-            const float b = vec_best<DT, LARGEST>(x[u]);
-            hb |= ((LARGEST ? (b >= tf) : (b <= tf)) ? 1u : 0u) << u;
+            const int32_t b = vec_best_ord<DT, LARGEST>(x[u]);
+            hb |= ((LARGEST ? (b >= to) : (b <= to)) ? 1u : 0u) << u;
           }
           if (__ballot(hb != 0)) {  // rare: exact per-element masks, only for
 #pragma unroll                      // the vector slots some lane hit
@@ -709,10 +706,7 @@ void k_filter(KParams p) {
               const Vec<DT> v = unpack<DT>(x[u]);
               uint32_t mm = 0;
 #pragma unroll
-              for (int c = 0; c < P; ++c) {
-                const float xf = bits_to_float<DT>(v.e[c]);
-                mm |= ((LARGEST ? (xf >= tf) : (xf <= tf)) ? 1u : 0u) << c;
-              }
+              for (int c = 0; c < P; ++c) mm |= (sel_key<DT, LARGEST>(v.e[c]) >= T ? 1u : 0u) << c;
               mm = (j < nvc && ((hb >> u) & 1u)) ? mm : 0u;
               if (__ballot(mm != 0)) append_vec<DT, LARGEST>(ac, staged, x[u], mm, al + (c0 + j) * P);
             }

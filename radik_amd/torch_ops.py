@@ -14,14 +14,23 @@ the MI355X top-k as ``torch.ops.radik_amd.*`` (registered on import):
       reference's [taskNum][K] output (radik/RadixSelect/topk_radixselect.h:312-320).
 
 Both run on the current stream without a host synchronisation (topk_ragged
-takes the host bound max_k so that it needs none).
+takes the host bound max_k so that it needs none).  Everything the host can
+check is checked before the launch (dtype, k against the dimension and the
+sorted limit, a dimension or slice count above 2^31 - 1).  What only the
+device sees (topk_ragged: a row whose k exceeds max_k, offsets that
+decrease; an internal capacity failure) leaves that row unwritten: topk_ragged
+pre-fills its outputs with value 0 / index -1, so such a row reads as empty.
+``checked=True`` waits for the call and raises RuntimeError if the device
+status of the call is not clean (one host synchronisation).
 """
 
 
 import torch
 
 from . import (RK_ASCEND, RK_LARGEST, RK_SORTED, _check, _dtype_code, lib, max_k as _max_k,
-               workspace_size)
+               status as _status, workspace_size)
+
+_INT32_MAX = 2 ** 31 - 1
 
 _SUPPORTED = (torch.float32, torch.float16, torch.bfloat16)
 
@@ -32,25 +41,35 @@ def _flags(largest: bool, sorted: bool) -> int:
         0 if largest else RK_ASCEND)
 
 
-def _run(values, offsets, ks, num_rows, total, max_k, flags, out_v, out_i, out_off):
+def _run(values, offsets, ks, num_rows, total, max_k, flags, out_v, out_i, out_off, checked,
+         op):
     dt = _dtype_code(values)
     ws = torch.empty(workspace_size(dt, num_rows, total, max_k, flags), dtype=torch.uint8,
                      device=values.device)
-    stream = torch.cuda.current_stream(values.device).cuda_stream
+    cur = torch.cuda.current_stream(values.device)
     rc = lib().rk_topk_batched(dt, values.data_ptr(), offsets.data_ptr(), ks.data_ptr(),
                                num_rows, total, max_k, flags, None, out_v.data_ptr(),
                                out_i.data_ptr(), out_off.data_ptr(), ws.data_ptr(), ws.numel(),
-                               stream)
+                               cur.cuda_stream)
     _check(rc, "rk_topk_batched")
+    if checked:
+        st = _status(ws, cur)
+        if st[0] != 0:
+            raise RuntimeError(f"radik_amd::{op}: device status 0x{st[0]:x} (bad row: 1, "
+                               f"workspace bound: 2 or 4, internal: 8); failed rows are "
+                               f"left unwritten")
 
 
 @torch.library.custom_op("radik_amd::topk", mutates_args=(), device_types="cuda")
 def topk(values: torch.Tensor, k: int, dim: int = -1, largest: bool = True,
-         sorted: bool = True) -> tuple[torch.Tensor, torch.Tensor]:
+         sorted: bool = True, checked: bool = False) -> tuple[torch.Tensor, torch.Tensor]:
     if values.dtype not in _SUPPORTED:
         raise TypeError(f"radik_amd::topk: unsupported dtype {values.dtype}")
     d = dim % max(values.dim(), 1)
     n = values.shape[d] if values.dim() else 1
+    if n > _INT32_MAX:
+        raise ValueError(f"radik_amd::topk: a dimension of size {n} exceeds 2^31 - 1 "
+                         f"(row positions are int32)")
     if k < 0 or k > n:
         raise ValueError(f"radik_amd::topk: k={k} out of range for a dimension of size {n}")
     if sorted and k > _max_k():
@@ -68,7 +87,8 @@ def topk(values: torch.Tensor, k: int, dim: int = -1, largest: bool = True,
         off = torch.arange(0, rows + 1, dtype=torch.int64, device=dev) * n
         oo = torch.arange(0, rows + 1, dtype=torch.int64, device=dev) * k
         kk = torch.full((rows,), k, dtype=torch.int32, device=dev)
-        _run(x, off, kk, rows, rows * n, k, _flags(largest, sorted), out_v, out_i32, oo)
+        _run(x, off, kk, rows, rows * n, k, _flags(largest, sorted), out_v, out_i32, oo,
+             checked, "topk")
     out_i = out_i32.to(torch.int64)
     if values.dim():
         out_v, out_i = out_v.movedim(-1, d), out_i.movedim(-1, d)
@@ -78,7 +98,7 @@ def topk(values: torch.Tensor, k: int, dim: int = -1, largest: bool = True,
 
 
 @topk.register_fake
-def _topk_fake(values, k, dim=-1, largest=True, sorted=True):
+def _topk_fake(values, k, dim=-1, largest=True, sorted=True, checked=False):
     shape = list(values.shape)
     if shape:
         shape[dim % len(shape)] = k
@@ -87,7 +107,8 @@ def _topk_fake(values, k, dim=-1, largest=True, sorted=True):
 
 @torch.library.custom_op("radik_amd::topk_ragged", mutates_args=(), device_types="cuda")
 def topk_ragged(values: torch.Tensor, offsets: torch.Tensor, ks: torch.Tensor, max_k: int,
-                largest: bool = True, sorted: bool = True) -> tuple[torch.Tensor, torch.Tensor]:
+                largest: bool = True, sorted: bool = True,
+                checked: bool = False) -> tuple[torch.Tensor, torch.Tensor]:
     if values.dtype not in _SUPPORTED or values.dim() != 1:
         raise TypeError("radik_amd::topk_ragged: values must be a 1-D float32/float16/bfloat16 "
                         "tensor")
@@ -98,17 +119,20 @@ def topk_ragged(values: torch.Tensor, offsets: torch.Tensor, ks: torch.Tensor, m
         raise ValueError(f"radik_amd::topk_ragged: max_k={max_k} out of range for "
                          f"{'sorted' if sorted else 'unsorted'} output")
     rows = ks.numel()
-    out_v = torch.empty((rows, max_k), dtype=values.dtype, device=values.device)
-    out_i = torch.empty((rows, max_k), dtype=torch.int32, device=values.device)
+    if rows > _INT32_MAX:
+        raise ValueError("radik_amd::topk_ragged: more than 2^31 - 1 rows")
+    # pre-filled (the tail fill), so a row the device rejects reads as empty
+    out_v = torch.zeros((rows, max_k), dtype=values.dtype, device=values.device)
+    out_i = torch.full((rows, max_k), -1, dtype=torch.int32, device=values.device)
     if rows and max_k:
         oo = torch.arange(0, rows + 1, dtype=torch.int64, device=values.device) * max_k
         _run(values.contiguous(), offsets.contiguous(), ks.contiguous(), rows, values.numel(),
-             max_k, _flags(largest, sorted), out_v, out_i, oo)
+             max_k, _flags(largest, sorted), out_v, out_i, oo, checked, "topk_ragged")
     return out_v, out_i
 
 
 @topk_ragged.register_fake
-def _topk_ragged_fake(values, offsets, ks, max_k, largest=True, sorted=True):
+def _topk_ragged_fake(values, offsets, ks, max_k, largest=True, sorted=True, checked=False):
     rows = ks.shape[0]
     return (values.new_empty((rows, max_k)),
             values.new_empty((rows, max_k), dtype=torch.int32))

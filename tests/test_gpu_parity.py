@@ -251,3 +251,149 @@ def test_repeat_calls_reuse_workspace_deterministic():
     for o in outs[1:]:
         np.testing.assert_array_equal(o[0], outs[0][0])
         np.testing.assert_array_equal(o[1], outs[0][1])
+
+
+# ---------------------------------------------------------------------------
+# Signed zeros, signalling NaNs and subnormals on the streaming filter path.
+# Rows are long (>= 70k elements, lengths multiples of 64 so that every row
+# starts on a 128-byte line) and their k is small, so they take the sampled
+# filter (or, with RK_EXACT, the exact passes and the same filter).  Values
+# are built as bit patterns; the smallest-k variants flip every sign bit,
+# which mirrors the row.
+# ---------------------------------------------------------------------------
+SIGN = {"f32": 0x80000000, "bf16": 0x8000, "f16": 0x8000}
+SNAN = {"f32": [0x7F800001, 0x7FBFFFFF, 0x7FA00000, 0xFF800001, 0xFFBFFFFF],
+        "bf16": [0x7F81, 0x7FBF, 0x7FA0, 0xFF81, 0xFFBF],
+        "f16": [0x7C01, 0x7DFF, 0x7D00, 0xFC01, 0xFDFF]}
+QNAN = {"f32": [0x7FC00000, 0xFFC00000], "bf16": [0x7FC0, 0xFFC0], "f16": [0x7E00, 0xFE00]}
+SUBN_MAX = {"f32": 0x007FFFFF, "bf16": 0x007F, "f16": 0x03FF}
+
+
+def _bits_of(dt, x):
+    """Bit patterns of float32 values x rounded to dt (f16 round to nearest;
+    bf16 truncated: the row contents only need to be known bit patterns)."""
+    x = np.asarray(x, np.float32)
+    if dt == "f32":
+        return x.view(np.uint32).copy()
+    if dt == "f16":
+        return x.astype(np.float16).view(np.uint16).copy()
+    return (x.view(np.uint32) >> 16).astype(np.uint16)
+
+
+def _normal_row(dt, rng, n, scale=1.0):
+    return _bits_of(dt, rng.standard_normal(n).astype(np.float32) * np.float32(scale))
+
+
+def _row_snan_next_to_candidates(dt, rng, n, ncand):
+    """N(0,1) row; ncand candidates (50, 50.25, ...) each alone in its
+    16-byte vector among signalling NaNs; 500 more vectors of NaNs only."""
+    P = 4 if dt == "f32" else 8
+    row = _normal_row(dt, rng, n)
+    vecs = rng.choice(n // P, ncand + 500, replace=False)
+    cand = _bits_of(dt, 50.0 + 0.25 * np.arange(ncand, dtype=np.float32))
+    nans = np.array(SNAN[dt] + QNAN[dt], dtype=row.dtype)
+    for j, v in enumerate(vecs):
+        lane = rng.integers(0, P)
+        for c in range(P):
+            row[v * P + c] = nans[rng.integers(0, len(SNAN[dt]))] if c != lane or j >= ncand \
+                else cand[j]
+        if j >= ncand:
+            row[v * P + lane] = nans[rng.integers(0, len(nans))]
+    return row
+
+
+def _row_subnormal_kth(dt, rng, n, k):
+    """Mostly negative normals; 40 positive normals, 3k distinct positive
+    subnormals (some repeated), +0 and -0: the k-th largest is a subnormal.
+    Salted with signalling NaNs."""
+    row = _normal_row(dt, rng, n)
+    row |= np.array(SIGN[dt], dtype=row.dtype)
+    pos = rng.choice(n, 40 + 3 * k + 400, replace=False)
+    row[pos[:40]] = _bits_of(dt, 1.0 + np.arange(40, dtype=np.float32))
+    sub = rng.choice(np.arange(1, SUBN_MAX[dt] + 1), 3 * k, replace=SUBN_MAX[dt] < 3 * k)
+    sub[: k // 4] = sub[k // 4: k // 2]  # repeats: ties at subnormal keys
+    row[pos[40:40 + 3 * k]] = sub.astype(row.dtype)
+    rest = pos[40 + 3 * k:]
+    row[rest[:100]] = 0
+    row[rest[100:300]] = SIGN[dt]
+    row[rest[300:]] = np.array(SNAN[dt], dtype=row.dtype)[rng.integers(0, 5, len(rest) - 300)]
+    return row
+
+
+def _row_zero_boundary(dt, rng, n, k, kth_negative_zero):
+    """The row of the +0/-0 boundary: 50 ones, then +0 and -0 filling the
+    rest.  With 60 +0 (kth_negative_zero False) the k-th largest (k = 100) is
+    +0 and every -0 must stay out; with 30 +0 it is -0 with many ties."""
+    dtype = np.uint32 if dt == "f32" else np.uint16
+    row = np.full(n, SIGN[dt], dtype)
+    pos = rng.choice(n, 50 + 60, replace=False)
+    row[pos[:50]] = _bits_of(dt, np.ones(50, np.float32))
+    row[pos[50:50 + (30 if kth_negative_zero else 60)]] = 0
+    return row
+
+
+def _mirror(dt, row):
+    return row ^ np.array(SIGN[dt], dtype=row.dtype)
+
+
+@pytest.mark.parametrize("dt", ["f32", "bf16", "f16"])
+@pytest.mark.parametrize("largest", [True, False])
+def test_signed_zero_boundary_long_rows(dt, largest):
+    """k-th value +0 in a row of -0 (and the mirror for smallest-k): the
+    filter must admit exactly the keys at or above the threshold, not every
+    signed zero (they would overflow the row's candidate slots)."""
+    rng = np.random.default_rng(91 + DT_CODE[dt])
+    n = 1 << 20
+    rows = [_row_zero_boundary(dt, rng, n, 100, False), _row_zero_boundary(dt, rng, n, 100, True)]
+    if not largest:
+        rows = [_mirror(dt, r) for r in rows]
+    bits = np.concatenate(rows)
+    off = np.array([0, n, 2 * n], np.int64)
+    ks = np.array([100, 100], np.int32)
+    d = LARGEST if largest else ASCEND
+    for flags in (d | SORTED, d):
+        for exact in (False, True):
+            _check(dt, bits, off, ks, flags | (EXACT if exact else 0))
+
+
+@pytest.mark.parametrize("dt", ["f32", "bf16", "f16"])
+@pytest.mark.parametrize("largest", [True, False])
+def test_signalling_nan_and_subnormal_long_rows(dt, largest):
+    """Signalling-NaN payloads in the same 16-byte vector as the candidates,
+    subnormal k-th values (with ties), rows of subnormals only, and the
+    +0/-0 boundary, through the sampled filter and the exact passes."""
+    rng = np.random.default_rng(71 + 3 * DT_CODE[dt])
+    rows = [
+        _row_snan_next_to_candidates(dt, rng, 70_016, 300),
+        _row_subnormal_kth(dt, rng, 131_072, 100),
+        _row_subnormal_kth(dt, rng, 75_008, 1000),
+        # subnormals and zeros only (the tiny scale underflows N(0,1))
+        _normal_row(dt, rng, 90_048, {"f32": 1e-39, "bf16": 1e-39, "f16": 1e-6}[dt]),
+        _row_zero_boundary(dt, rng, 80_000, 100, False),
+    ]
+    ks = np.array([200, 100, 1000, 1500, 100], np.int32)
+    if not largest:
+        rows = [_mirror(dt, r) for r in rows]
+    lens = np.array([len(r) for r in rows], np.int64)
+    off = np.zeros(len(rows) + 1, np.int64)
+    off[1:] = np.cumsum(lens)
+    bits = np.concatenate(rows)
+    # best first, worst first, position order
+    flag_sets = ((LARGEST | SORTED, LARGEST | SORTED | ASCEND, LARGEST) if largest else
+                 (SORTED | ASCEND, SORTED, 0))
+    for flags in flag_sets:
+        for exact in (False, True):
+            _check(dt, bits, off, ks, flags | (EXACT if exact else 0))
+
+
+@pytest.mark.parametrize("flags", [LARGEST | SORTED, LARGEST])
+@pytest.mark.parametrize("exact", [False, True])
+def test_workload_c1_full_size(flags, exact):
+    """Config 1 at its full size: one row of 2^20 fp32 U[0,1) on the 2^-24
+    grid (seed 1), k = 256, largest; sorted and unsorted, sampled and exact."""
+    import radik_amd.workloads as W
+    w = W.c1()
+    vals = W.materialize(w)
+    bits = from_device_bits(vals, "f32")[: w.total]
+    st = _check("f32", bits, w.offsets(), w.ks, flags | (EXACT if exact else 0))
+    assert st[0] == 0

@@ -132,11 +132,11 @@ def test_read_refuses_a_file_larger_than_the_buffers(tmp_path):
     ks = np.full(1, -5, np.int32)
     v = np.full(4, 7, np.uint32)
     args = (off.ctypes.data_as(vp), ks.ctypes.data_as(vp), v.ctypes.data_as(vp))
-    assert L.rk_batch_read(big.encode(), 1, 4, *args) == -7
+    assert L.rk_batch_read(big.encode(), rk.RK_F32, 1, 4, *args) == -7
     assert list(off) == [-5, -5] and list(ks) == [-5] and list(v) == [7] * 4
-    assert L.rk_batch_read(small.encode(), 1, 4, *args) == 0
+    assert L.rk_batch_read(small.encode(), rk.RK_F32, 1, 4, *args) == 0
     assert list(off) == [0, 4] and list(ks) == [2] and list(v) == [0, 1, 2, 3]
-    assert L.rk_batch_read(small.encode(), -1, 4, *args) == -1
+    assert L.rk_batch_read(small.encode(), rk.RK_F32, -1, 4, *args) == -1
 
     rs, rb = str(tmp_path / "s.rkr"), str(tmp_path / "b.rkr")
     rk.save_results(rs, rk.RK_F32, [0, 2], np.ones(2, np.uint32), np.zeros(2, np.int32))
@@ -145,7 +145,40 @@ def test_read_refuses_a_file_larger_than_the_buffers(tmp_path):
     rv = np.full(2, 7, np.uint32)
     ri = np.full(2, 7, np.int32)
     rargs = (ro.ctypes.data_as(vp), rv.ctypes.data_as(vp), ri.ctypes.data_as(vp))
-    assert L.rk_result_read(rb.encode(), 1, 2, *rargs) == -7
+    assert L.rk_result_read(rb.encode(), rk.RK_F32, 1, 2, *rargs) == -7
     assert list(ro) == [-5, -5] and list(rv) == [7, 7]
-    assert L.rk_result_read(rs.encode(), 1, 2, *rargs) == 0
+    assert L.rk_result_read(rs.encode(), rk.RK_F32, 1, 2, *rargs) == 0
     assert list(ro) == [0, 2] and list(rv) == [1, 1] and list(ri) == [0, 0]
+
+
+def test_read_refuses_a_file_whose_dtype_changed(tmp_path):
+    """A file rewritten with 4-byte values after *_info reported 2-byte ones
+    (same rows and totals) would overflow buffers sized for the old dtype:
+    *_read takes the dtype it sized for and refuses the file untouched."""
+    L = rk.lib()
+    vp = rk.ctypes.c_void_p
+    p = str(tmp_path / "x.rkb")
+    rk.save_batch(p, rk.RK_BF16, np.arange(4, dtype=np.uint16), [0, 4], [2])
+    info = rk._FileInfo()
+    assert L.rk_batch_info(p.encode(), rk.ctypes.byref(info)) == 0
+    assert info.dtype == rk.RK_BF16
+    rk.save_batch(p, rk.RK_F32, np.arange(4, dtype=np.uint32), [0, 4], [2])  # swapped
+    off = np.full(2, -5, np.int64)
+    ks = np.full(1, -5, np.int32)
+    v = np.full(4 + 8, 7, np.uint16)  # sized for 4 bf16 values, plus a guard
+    args = (off.ctypes.data_as(vp), ks.ctypes.data_as(vp), v.ctypes.data_as(vp))
+    assert L.rk_batch_read(p.encode(), info.dtype, info.rows, info.total, *args) == -7
+    assert list(off) == [-5, -5] and list(ks) == [-5] and list(v) == [7] * 12
+    assert L.rk_batch_read(p.encode(), rk.RK_F32, 1, 4, *args) == 0
+    assert L.rk_batch_read(p.encode(), 9, 1, 4, *args) == -3
+
+    r = str(tmp_path / "x.rkr")
+    rk.save_results(r, rk.RK_F16, [0, 2], np.ones(2, np.uint16), np.zeros(2, np.int32))
+    assert L.rk_result_info(r.encode(), rk.ctypes.byref(info)) == 0
+    rk.save_results(r, rk.RK_F32, [0, 2], np.ones(2, np.uint32), np.zeros(2, np.int32))
+    ro = np.full(2, -5, np.int64)
+    rv = np.full(2 + 4, 7, np.uint16)
+    ri = np.full(2, 7, np.int32)
+    rargs = (ro.ctypes.data_as(vp), rv.ctypes.data_as(vp), ri.ctypes.data_as(vp))
+    assert L.rk_result_read(r.encode(), info.dtype, info.rows, info.total, *rargs) == -7
+    assert list(ro) == [-5, -5] and list(rv) == [7] * 6 and list(ri) == [7, 7]

@@ -111,3 +111,42 @@ def test_topk_ragged_matches_oracle():
                       flags=orc.LARGEST | orc.SORTED, out_off=oo, nthreads=8)
     np.testing.assert_array_equal(i.reshape(-1).cpu().numpy(), ei)
     np.testing.assert_array_equal(from_device_bits(v.reshape(-1), "f32"), eb)
+
+
+@pytest.mark.gpu
+def test_device_rejected_rows_read_empty_and_checked_raises():
+    """A row the device rejects (its k above the max_k bound the host passed)
+    is left at the pre-fill, value 0 / index -1, while the other rows are
+    exact; checked=True reports the device status as RuntimeError."""
+    import torch
+
+    import radik_amd.torch_ops  # noqa: F401
+    x = _distinct((3, 2000), torch.float32, 4).reshape(-1)
+    off = torch.tensor([0, 2000, 4000, 6000], dtype=torch.int64, device="cuda")
+    ks = torch.tensor([5, 50, 7], dtype=torch.int32, device="cuda")  # row 1: k > max_k
+    v, i = torch.ops.radik_amd.topk_ragged(x, off, ks, 10)
+    assert torch.equal(i[1], torch.full((10,), -1, dtype=torch.int32, device="cuda"))
+    assert torch.equal(v[1], torch.zeros(10, device="cuda"))
+    rv, ri = torch.topk(x[:2000], 5)
+    assert torch.equal(i[0, :5].long(), ri) and torch.equal(i[0, 5:], i[1, 5:])
+    rv, ri = torch.topk(x[4000:], 7)
+    assert torch.equal(i[2, :7].long(), ri)
+    with pytest.raises(RuntimeError, match="device status"):
+        torch.ops.radik_amd.topk_ragged(x, off, ks, 10, checked=True)
+    ks[1] = 9
+    v2, i2 = torch.ops.radik_amd.topk_ragged(x, off, ks, 10, checked=True)
+    assert torch.equal(i2[0], i[0]) and torch.equal(i2[2], i[2])
+    v3, i3 = torch.ops.radik_amd.topk(x.reshape(3, 2000), 4, checked=True)
+    assert torch.equal(i3, torch.topk(x.reshape(3, 2000), 4).indices)
+
+
+@pytest.mark.gpu
+def test_topk_refuses_a_dimension_above_int32():
+    """Row positions are int32: a dimension of 2^31 elements is refused on
+    the host (a stride-0 view, so nothing of that size is allocated)."""
+    import torch
+
+    import radik_amd.torch_ops  # noqa: F401
+    big = torch.zeros(1, device="cuda").expand(2 ** 31)
+    with pytest.raises(ValueError, match="2\\^31"):
+        torch.ops.radik_amd.topk(big, 4)
