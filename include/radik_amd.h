@@ -49,7 +49,8 @@ enum {
   RK_E_ALIGN = -4,       /* value pointer not aligned to its element size */
   RK_E_HIP = -5,         /* a HIP runtime call failed */
   RK_E_DEVICE = -6,      /* device-side validation failed (rk_topk_status) */
-  RK_E_IO = -7           /* batch / result file missing, unreadable or malformed */
+  RK_E_IO = -7,          /* batch / result file missing, unreadable or malformed */
+  RK_E_SCRATCH = -8      /* call too large for rk_topk_batched_noalloc's scratch */
 };
 
 /* Largest k for sorted output on the current device (16384 on MI355X: the
@@ -83,6 +84,26 @@ int rk_topk_batched(int dtype, const void* vals, const int64_t* row_off,
                     int32_t max_k, uint32_t flags, const int32_t* idx_in,
                     void* out_vals, int32_t* out_idx, const int64_t* out_off,
                     void* workspace, size_t ws_bytes, void* stream);
+
+/* Allocation-free form of rk_topk_batched.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory or hand in
+ * a workspace (code under hipGraph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments and results as rk_topk_batched,
+ * without the workspace: the call's scratch is a device array of
+ * RK_NOALLOC_SCRATCH_BYTES built into the library, one per GPU, held from the
+ * library's load on that GPU until the process ends.  A call whose
+ * rk_topk_workspace_size exceeds it returns RK_E_SCRATCH before any device
+ * work.  The call allocates nothing and is graph-capturable.  All calls on
+ * one GPU share that array: calls that overlap on one GPU (two streams or two
+ * host threads without ordering between them) race, so serialize them.  The
+ * device status of these calls is not exposed; a row the device rejects
+ * (k above max_k, bad offsets) is left unwritten. */
+#define RK_NOALLOC_SCRATCH_BYTES (64u << 20)
+int rk_topk_batched_noalloc(int dtype, const void* vals, const int64_t* row_off,
+                            const int32_t* row_k, int32_t num_rows, int64_t total_len,
+                            int32_t max_k, uint32_t flags, const int32_t* idx_in,
+                            void* out_vals, int32_t* out_idx, const int64_t* out_off,
+                            void* stream);
 
 /* Same, with each row and its output at their own device addresses
  * (row_ptrs, row_len, row_k, out_val_ptrs and out_idx_ptrs are device

@@ -38,6 +38,8 @@ RK_KNN_IP, RK_KNN_L2 = 0, 1
 GEN_UNIFORM, GEN_NORMAL, GEN_ZIPF, GEN_NEARCONST, GEN_RANGE, GEN_CONST = range(6)
 RK_OK, RK_E_ARG, RK_E_WORKSPACE, RK_E_UNSUPPORTED, RK_E_ALIGN, RK_E_HIP, RK_E_DEVICE = (
     0, -1, -2, -3, -4, -5, -6)
+RK_E_IO, RK_E_SCRATCH = -7, -8
+RK_NOALLOC_SCRATCH_BYTES = 64 << 20  # include/radik_amd.h
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.environ.get("RADIK_AMD_LIB") or os.path.join(_HERE, "lib", "libradik_amd.so")
@@ -48,11 +50,13 @@ _ERRORS = {
     -3: "unsupported dtype, or sorted output with k above rk_max_k()", -4: "misaligned pointer",
     -5: "HIP runtime error", -6: "device-side validation failed",
     -7: "batch or result file missing, unreadable or malformed",
+    -8: "call too large for the allocation-free entry's scratch",
 }
 
 # Every symbol include/radik_amd.h and include/radik_amd_harness.h declare.
 EXPORTED = (
-    "rk_max_k", "rk_topk_workspace_size", "rk_topk_batched", "rk_topk_rows",
+    "rk_max_k", "rk_topk_workspace_size", "rk_topk_batched", "rk_topk_batched_noalloc",
+    "rk_topk_rows",
     "rk_compat_workspace_size", "rk_topk_compat", "rk_topk_status", "rk_strerror",
     "rk_get_tune", "rk_set_tune", "rk_gen_fill", "rk_timing_enable", "rk_timing_read",
     "rk_autotune_batched", "rk_tune_cache_save", "rk_tune_cache_load", "rk_tune_cache_clear",
@@ -122,6 +126,8 @@ def lib():
     L.rk_topk_workspace_size.argtypes = [c.c_int, i32, i64, i32, u32, c.POINTER(sz)]
     L.rk_topk_batched.argtypes = [c.c_int, vp, vp, vp, i32, i64, i32, u32, vp, vp, vp, vp,
                                   vp, sz, vp]
+    L.rk_topk_batched_noalloc.argtypes = [c.c_int, vp, vp, vp, i32, i64, i32, u32, vp, vp, vp,
+                                          vp, vp]
     L.rk_topk_rows.argtypes = [c.c_int, vp, vp, vp, i32, i64, i32, u32, vp, vp, vp, sz, vp]
     L.rk_compat_workspace_size.argtypes = [c.c_int, i32, i32, i32, u32, c.POINTER(sz)]
     L.rk_topk_compat.argtypes = [c.c_int, vp, vp, vp, vp, vp, sz, vp, i32, i32, u32, vp]

@@ -251,6 +251,38 @@ int topk_rows_part(int dtype, const void* const* row_ptrs, const int64_t* row_le
 
 }  // namespace rk
 
+// ---------------------------------------------------------------------------
+// Allocation-free entry.  Its workspace is this file-scope device array, one
+// copy per GPU (the code object's storage on that GPU, held until the
+// process ends).  In host code the array's name is only a host-side stand-in:
+// the device address is taken once per device with hipGetSymbolAddress, with
+// that device current, and only that address is ever passed to a kernel.
+// Nothing in it must start at zero: the plan kernel writes every header
+// field, row state, counter and table entry a later kernel reads, so the
+// previous call's contents are harmless.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[RK_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+// Device address of `scratch` on the current device (null on a HIP error).
+void* scratch_address() {
+  static std::mutex mu;
+  static std::vector<void*> addr;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int)addr.size() <= dev) addr.resize(dev + 1, nullptr);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+    addr[dev] = p;
+  }
+  return addr[dev];
+}
+
+}  // namespace
+
 extern "C" {
 
 int32_t rk_max_k(void) {
@@ -338,6 +370,25 @@ int rk_topk_compat(int dtype, const void* valIn, const int32_t* idxIn, void* val
                 ws_bytes - off_bytes, stream);
 }
 
+int rk_topk_batched_noalloc(int dtype, const void* vals, const int64_t* row_off,
+                            const int32_t* row_k, int32_t num_rows, int64_t total_len,
+                            int32_t max_k, uint32_t flags, const int32_t* idx_in, void* out_vals,
+                            int32_t* out_idx, const int64_t* out_off, void* stream) {
+  if (!valid_dtype(dtype)) return RK_E_UNSUPPORTED;
+  if (!row_k || !out_off) return RK_E_ARG;
+  if (num_rows < 0 || total_len < 0 || max_k < 0) return RK_E_ARG;
+  if (make_layout(num_rows, total_len, max_k).total > (size_t)RK_NOALLOC_SCRATCH_BYTES)
+    return RK_E_SCRATCH;
+  void* ws = scratch_address();
+  if (!ws) return RK_E_HIP;
+  Outputs o;
+  o.vals = out_vals;
+  o.idx = out_idx;
+  o.off = out_off;
+  return launch(dtype, vals, row_off, nullptr, nullptr, row_k, 0, num_rows, total_len, max_k,
+                flags, idx_in, o, ws, (size_t)RK_NOALLOC_SCRATCH_BYTES, stream);
+}
+
 int rk_topk_status(const void* workspace, void* stream, uint32_t* status, uint32_t* fallback_rows,
                    uint32_t* ordered_rows) {
   if (!workspace) return RK_E_WORKSPACE;
@@ -362,6 +413,9 @@ const char* rk_strerror(int code) {
     case RK_E_HIP: return "HIP runtime error";
     case RK_E_DEVICE: return "device-side validation failed";
     case RK_E_IO: return "batch or result file missing, unreadable or malformed";
+    case RK_E_SCRATCH:
+      return "call needs more workspace than the allocation-free entry's "
+             "RK_NOALLOC_SCRATCH_BYTES";
     default: return "unknown error";
   }
 }
