@@ -60,8 +60,8 @@ DevInfo device_info() {
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct Layout {
-  size_t hdr, desc, st, tile_start, cand_base, cand_cnt, order, ghist, tile_cnt, tile_pre,
-      chunk_row, cand, total;
+  size_t hdr, desc, st, tile_start, cand_base, cand_cnt, order, ghist, tile_cnt, chunk_row,
+      cand, total;
   uint64_t tile_cap, cand_cap;
 };
 
@@ -87,7 +87,6 @@ Layout make_layout(int32_t R, int64_t total_len, int32_t max_k) {
   L.order = o; o = align256(o + 2 * Ru * sizeof(uint32_t));
   L.ghist = o; o = align256(o + Ru * rk::HIST_MAX_BINS * sizeof(uint32_t));
   L.tile_cnt = o; o = align256(o + L.tile_cap * sizeof(uint2));
-  L.tile_pre = o; o = align256(o + L.tile_cap * sizeof(uint2));
   L.chunk_row = o; o = align256(o + L.tile_cap * sizeof(uint32_t));
   L.cand = o; o = align256(o + L.cand_cap * sizeof(uint64_t));
   L.total = o;
@@ -186,7 +185,6 @@ int launch(int dtype, const void* vals, const int64_t* row_off, const void* cons
   p.order = reinterpret_cast<uint32_t*>(ws + L.order);
   p.ghist = reinterpret_cast<uint32_t*>(ws + L.ghist);
   p.tile_cnt = reinterpret_cast<uint2*>(ws + L.tile_cnt);
-  p.tile_pre = reinterpret_cast<uint2*>(ws + L.tile_pre);
   p.chunk_row = reinterpret_cast<uint32_t*>(ws + L.chunk_row);
   p.cand = reinterpret_cast<uint64_t*>(ws + L.cand);
   p.tile_capacity = L.tile_cap;

@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #define RK_DEV __device__ __forceinline__
+#define RK_HOST_DEV __host__ __device__ __forceinline__
 
 namespace rk {
 
@@ -488,6 +489,63 @@ RK_DEV void bitonic_sort_e(uint64_t* a, uint32_t P2) {
   for (int e = 0; e < E; ++e)
     if (t < nthr) a[t * E + e] = v[e];
   __syncthreads();
+}
+
+// Descending bitonic sort of NT * E 64-bit keys held in registers, thread t
+// holding keys [t E, t E + E) (NT a multiple of 64).  A stage at distance
+// `stride` runs in registers when stride < E, by lane shuffles when the
+// partner thread is in the same wave (stride / E < 64), and through `xs`
+// (LDS, NT * E keys, laid out key-major so that a wave's accesses fall in
+// distinct banks) otherwise: of the 78 stages of 4096 keys in 256 x 16,
+// three cross waves.  No barrier except around those stages.
+template <int NT, int E>
+RK_DEV void bitonic_sort_regs(uint64_t (&v)[E], uint64_t* xs) {
+  constexpr uint32_t N = (uint32_t)NT * E;
+  const uint32_t t = threadIdx.x;
+  for (uint32_t size = 2; size <= N; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride < (uint32_t)E) {
+#pragma unroll
+        for (int sc = E / 2; sc >= 1; sc >>= 1) {
+          if (stride == (uint32_t)sc) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+              if (!(e & sc)) {
+                const bool desc = !((t * E + e) & size);
+                const uint64_t x = v[e], y = v[e + sc];
+                const bool sw = desc ? (x < y) : (x > y);
+                v[e] = sw ? y : x;
+                v[e + sc] = sw ? x : y;
+              }
+            }
+          }
+        }
+      } else {
+        // All E partner keys are fetched before any is used, so the E
+        // exchanges of a stage are in flight together.
+        const uint32_t ls = stride / E;  // distance in threads
+        uint64_t y[E];
+        if (ls < 64) {
+#pragma unroll
+          for (int e = 0; e < E; ++e) y[e] = __shfl_xor(v[e], (int)ls, 64);
+        } else {
+#pragma unroll
+          for (int e = 0; e < E; ++e) xs[(uint32_t)e * NT + t] = v[e];
+          __syncthreads();
+#pragma unroll
+          for (int e = 0; e < E; ++e) y[e] = xs[(uint32_t)e * NT + (t ^ ls)];
+          __syncthreads();
+        }
+        // i & stride and i & size are the same for all E keys of a thread
+        // here (stride >= E), so one direction per thread
+        const uint32_t i0 = t * E;
+        const bool keep_max = (!(i0 & stride)) == (!(i0 & size));
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+          v[e] = keep_max ? (v[e] > y[e] ? v[e] : y[e]) : (v[e] < y[e] ? v[e] : y[e]);
+      }
+    }
+  }
 }
 
 // Compare-exchange of x (lower index) and y in a bitonic stage.

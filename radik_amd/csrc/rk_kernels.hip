@@ -9,11 +9,13 @@
 //   sample                          long rows: threshold T from a sample
 //   filter                          one streaming read: compact keys >= T
 //   select_finalize                 LDS radix select + bitonic sort + write
-//   exact_init, (hist, hist_select) x passes
+//   merge_sort, merge_rank          (max_k > 4096) rows with k > 4096: the
+//                                   sort split over workgroups
+//   [exact_init], (hist, hist_select) x passes
 //                                   exact RadiK passes for rows the sampled
 //                                   path could not settle (and the plain path)
 //   filter, select_finalize         second round for those rows
-//   emit_count, emit_scan, emit_write, finalize_ordered
+//   emit_count, emit_write, finalize_ordered
 //                                   rows whose ties at the k-th key overflow
 //                                   the candidate buffer: ordered emit
 //
@@ -225,8 +227,17 @@ RK_DEV int64_t aligned_start(const typename Traits<DT>::U* rb, int64_t es) {
 // ---------------------------------------------------------------------------
 // plan
 // ---------------------------------------------------------------------------
+// What the scan of k_plan_scan needs of a planned row (kept in registers by
+// the fused plan, so it makes no second round trip to memory).
+struct RowPlan {
+  int64_t len;
+  int32_t k;
+  uint32_t tiles, cap, phase;
+  bool bad;
+};
+
 template <int DT>
-RK_DEV void plan_row(const KParams& p, int r) {
+RK_DEV RowPlan plan_row(const KParams& p, int r) {
   using U = typename Traits<DT>::U;
   const U* base;
   const int32_t* idx = nullptr;
@@ -296,6 +307,7 @@ RK_DEV void plan_row(const KParams& p, int r) {
   p.tile_start[r] = tiles;
   p.cand_base[r] = s.cap;
   p.cand_cnt[r] = 0;
+  return RowPlan{len, (int32_t)k, tiles, s.cap, s.phase, bad};
 }
 
 template <int DT>
@@ -310,10 +322,17 @@ __global__ __launch_bounds__(256) void k_plan_rows(KParams p) {
 // saves a launch on small batches.
 template <int DT, bool FUSED>
 __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
-  if (FUSED) {
-    if ((int)threadIdx.x < p.R) plan_row<DT>(p, threadIdx.x);
-    __syncthreads();
-  }
+  // FUSED: thread t plans row t and keeps its plan in registers; otherwise
+  // the plans are read back from what k_plan_rows wrote.
+  RowPlan mine{0, 0, 0u, 0u, (uint32_t)PH_DONE, false};
+  uint32_t my_t0 = 0;
+  if (FUSED && (int)threadIdx.x < p.R) mine = plan_row<DT>(p, threadIdx.x);
+  auto plan_of = [&](int r) -> RowPlan {
+    if (FUSED) return mine;  // r == threadIdx.x in every loop below
+    const RowState st = p.st[r];
+    return RowPlan{p.desc[r].len, p.desc[r].k, p.tile_start[r], st.cap, st.phase,
+                   (st.flags & 0x80000000u) != 0};
+  };
   __shared__ uint32_t scratch[16];
   __shared__ uint64_t scratch64[16];
   __shared__ uint32_t bad_sh;
@@ -329,10 +348,12 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
   uint64_t carry_c = 0;
   for (int base = 0; base < p.R; base += 1024) {
     const int r = base + tid;
-    const uint32_t t = r < p.R ? p.tile_start[r] : 0u;
-    const uint64_t c = r < p.R ? p.cand_base[r] : 0ull;
-    if (r < p.R && (p.st[r].flags & 0x80000000u)) atomicOr(&bad_sh, 1u);
-    if (r < p.R && p.st[r].phase == PH_HIST) atomicAdd(&hist_sh, 1u);
+    RowPlan rp{0, 0, 0u, 0u, (uint32_t)PH_DONE, false};
+    if (r < p.R) rp = plan_of(r);
+    const uint32_t t = rp.tiles;
+    const uint64_t c = rp.cap;
+    if (rp.bad) atomicOr(&bad_sh, 1u);
+    if (rp.phase == PH_HIST) atomicAdd(&hist_sh, 1u);
     uint32_t tot_t;
     const uint32_t ex_t = block_excl_scan<1024>(t, scratch, tot_t);
     // 64-bit scan of the capacities: wave scan + per-wave totals.
@@ -356,6 +377,7 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
       p.tile_start[r] = carry_t + ex_t;
       p.cand_base[r] = carry_c + pre + inc - c;
     }
+    if (FUSED) my_t0 = carry_t + ex_t;  // this row's first tile
     carry_t += tot_t;
     carry_c += tot_c;
   }
@@ -390,8 +412,9 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
     if (tid < 64) bucket[tid >> 5][tid & 31] = 0;
     __syncthreads();
     auto classes = [&](int r, uint32_t& kc, uint32_t& lc) {
-      const uint32_t k = (uint32_t)max(p.desc[r].k, 1);
-      const uint64_t len = (uint64_t)max<int64_t>(p.desc[r].len, 1);
+      const RowPlan rp = FUSED ? mine : RowPlan{p.desc[r].len, p.desc[r].k, 0u, 0u, 0u, false};
+      const uint32_t k = (uint32_t)max(rp.k, 1);
+      const uint64_t len = (uint64_t)max<int64_t>(rp.len, 1);
       kc = (uint32_t)min(31 - __builtin_clz(k), 15);
       lc = (uint32_t)min(63 - __builtin_clzll(len), 31);
     };
@@ -421,7 +444,7 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
   // On a capacity overflow no long row is processed (the status reports it).
   if (status_sh & (ST_TILES | ST_CANDIDATES)) {
     for (int r = tid; r < p.R; r += 1024) {
-      const uint32_t ph = p.st[r].phase;
+      const uint32_t ph = FUSED ? mine.phase : p.st[r].phase;
       if (ph == PH_SAMPLE || ph == PH_HIST) p.st[r].phase = PH_DONE;
       p.tile_start[r] = 0;
     }
@@ -429,13 +452,35 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
   }
   // Chunk -> row table of the filter: cleared, then filled for the rows that
   // skip sampling (the sample kernel fills it for the sampled rows).
-  const ChunkGrid g(p.tile_start[p.R], (uint32_t)p.chunk_tiles);
+  const ChunkGrid g(carry_t, (uint32_t)p.chunk_tiles);  // carry_t: total tiles (uniform)
   for (uint32_t c = tid; c < g.nch; c += 1024) p.chunk_row[c] = NO_ROW;
   __syncthreads();
   for (int r = tid; r < p.R; r += 1024) {
-    if (p.st[r].phase != PH_FILTER) continue;
-    const uint32_t c0 = g.first_ge(p.tile_start[r]), c1 = g.first_ge(p.tile_start[r + 1]);
+    if ((FUSED ? mine.phase : p.st[r].phase) != PH_FILTER) continue;
+    const uint32_t t0 = FUSED ? my_t0 : p.tile_start[r];
+    const uint32_t t1 = FUSED ? my_t0 + mine.tiles : p.tile_start[r + 1];
+    const uint32_t c0 = g.first_ge(t0), c1 = g.first_ge(t1);
     for (uint32_t c = c0; c < c1; ++c) p.chunk_row[c] = (uint32_t)r;
+  }
+}
+
+// Sends row r to the exact passes from its first digit: histogram cleared,
+// nothing resolved, all of its k still to take.  Called by a whole workgroup
+// of NT threads of the kernel that makes the decision, so the fallback round
+// needs no initialisation launch of its own.
+template <int NT>
+RK_DEV void exact_start_row(const KParams& p, int r, uint32_t why) {
+  uint32_t* h = p.ghist + (size_t)r * HIST_MAX_BINS;
+  for (int i = threadIdx.x; i < HIST_MAX_BINS; i += NT) h[i] = 0;
+  if (threadIdx.x == 0) {
+    RowState& w = p.st[r];
+    w.prefix = 0;
+    w.resolved = 0;
+    w.rem = (uint32_t)p.desc[r].k;
+    w.flags = 0;
+    w.why = why;
+    w.phase = PH_HIST;
+    p.cand_cnt[r] = 0;
   }
 }
 
@@ -463,7 +508,13 @@ __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
     __syncthreads();
     if (qi >= (uint32_t)p.R) break;
     const int r = (int)p.order[p.R + qi];
-    if (p.st[r].phase != PH_SAMPLE) continue;
+    const RowState st0 = p.st[r];
+    if (st0.phase == PH_HIST && (st0.flags & RF_NEED_INIT)) {
+      exact_start_row<SEL_NT>(p, r, st0.why);  // planned straight into the exact passes
+      __syncthreads();
+      continue;
+    }
+    if (st0.phase != PH_SAMPLE) continue;
     record_chunk_rows<SEL_NT>(p, r);
     RK_STAMP(p, r, 0);
     const RowDesc d = p.desc[r];
@@ -477,6 +528,21 @@ __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
     if (ng < ng_min) ng = ng_min;                   // but at least 64 runs,
     if (ng > ng_cap) ng = ng_cap;                   // and never beyond the LDS
     if (ng < 1) ng = 1;
+    // A smaller sample where it suffices: the runs are halved while the
+    // candidates expected at the resulting threshold, len * r(s) / s, stay
+    // within max(2k, k + 2048) and within half the select's LDS keys (fewer
+    // keys to load and select per row; e.g. one 2^20 row with k = 256 samples
+    // 4096 keys, expecting ~1800 candidates).
+    {
+      const float kf = (float)d.k, lim = fminf(fmaxf(2.0f * kf, kf + 2048.0f), 0.5f * p.lds_keys);
+      while (ng >= 128) {
+        const float s2 = (float)((ng / 2) * ge);
+        const float mu2 = s2 * kf / (float)len;
+        const float r2 = ceilf(mu2 + 4.0f * sqrtf(mu2) + 2.0f);
+        if ((float)len * r2 / s2 >= lim) break;
+        ng /= 2;
+      }
+    }
     const uint32_t s = (uint32_t)(ng * ge);
     // run g starts at floor(g * (len - ge) / (ng - 1)) (in double: no 64-bit
     // integer division per key), clamped so that the run stays in the row
@@ -511,20 +577,18 @@ __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
     uint32_t ge_cnt = 0;
     for (uint32_t i = threadIdx.x; i < s; i += SEL_NT) ge_cnt += (keys[i] >= T) ? 1u : 0u;
     ge_cnt = block_sum<SEL_NT>(ge_cnt, scratch);
-    if (threadIdx.x == 0) {
+    // Expected candidates: the sampled fraction at or above T, scaled.
+    const double est = (double)ge_cnt * (double)len / (double)s;
+    const uint32_t why = (T == 0 || rank >= s) ? FB_SAMPLE_DEGENERATE
+                         : est * 1.25 > (double)st0.cap ? FB_SAMPLE_TOO_MANY : FB_NONE;
+    if (why != FB_NONE) {  // uniform
+      exact_start_row<SEL_NT>(p, r, why);
+      if (threadIdx.x == 0) atomicAdd(&p.hdr->fallback_rows, 1u);
+    } else if (threadIdx.x == 0) {
       RowState& st = p.st[r];
-      // Expected candidates: the sampled fraction at or above T, scaled.
-      const double est = (double)ge_cnt * (double)len / (double)s;
-      if (T == 0 || rank >= s || est * 1.25 > (double)st.cap) {
-        st.phase = PH_HIST;
-        st.flags = RF_NEED_INIT;
-        st.why = (T == 0 || rank >= s) ? FB_SAMPLE_DEGENERATE : FB_SAMPLE_TOO_MANY;
-        atomicAdd(&p.hdr->fallback_rows, 1u);
-      } else {
-        st.T = T;
-        st.phase = PH_FILTER;
-        st.flags = 0;
-      }
+      st.T = T;
+      st.phase = PH_FILTER;
+      st.flags = 0;
     }
     __syncthreads();
   }
@@ -837,8 +901,10 @@ RK_DEV void sort_and_write(const KParams& p, const RowDesc& d, uint64_t* sbuf, u
 // largest composite keys, sort and write.  A sampled row that fails the
 // verification goes to the exact passes.
 // ---------------------------------------------------------------------------
+// At most 80 VGPRs (6 waves per SIMD): three 512-thread workgroups per CU.
 template <int DT, bool LARGEST>
-__global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
+__global__ __launch_bounds__(SEL_NT) __attribute__((amdgpu_waves_per_eu(6)))
+void k_select_finalize(KParams p) {
   if (p.round && p.hdr->fallback_rows == 0) return;
   using U = typename Traits<DT>::U;
   extern __shared__ uint64_t sbuf[];
@@ -858,6 +924,7 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
     const int r = (int)p.order[qi];
     const uint32_t ph = p.st[r].phase;
     if (ph != PH_COPYALL && ph != PH_SMALL && ph != PH_FILTER) continue;
+    uint32_t next_phase = PH_DONE;
     RK_STAMP(p, r, 8);
     const RowDesc d = p.desc[r];
     const U* rb = static_cast<const U*>(d.base);
@@ -910,7 +977,7 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
               ge += ((uint32_t)(ld_lds(keys + i) >> 32) >= st.T) ? 1u : 0u;
           } else {
             keys = src;
-            constexpr int UN = 4;  // loads in flight per thread
+            constexpr int UN = 8;  // loads in flight per thread
             for (uint32_t i0 = threadIdx.x; i0 < n; i0 += SEL_NT * UN) {
               uint64_t x[UN];
 #pragma unroll
@@ -928,12 +995,10 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
           ge = block_sum<SEL_NT>(ge, scratch);
           ok = ge >= k;
         }
-        if (!ok) {
+        if (!ok) {  // uniform
+          exact_start_row<SEL_NT>(p, r, why);
           if (threadIdx.x == 0) {
             if (st.flags & RF_EXACT_T) atomicOr(&p.hdr->status, (uint32_t)ST_INTERNAL);
-            p.st[r].phase = PH_HIST;
-            p.st[r].flags = RF_NEED_INIT;
-            p.st[r].why = why;
             atomicAdd(&p.hdr->fallback_rows, 1u);
           }
           __syncthreads();
@@ -946,6 +1011,11 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
           in_lds ? block_radix_select<SEL_NT, 11, uint64_t, false>(keys, n, k, hist, scratch, scratch64, outv)
                  : block_radix_select<SEL_NT, 11, uint64_t, true>(keys, n, k, hist, scratch, scratch64, outv);
       RK_STAMP(p, r, 10);
+      // A large selection (k > MERGE_CHUNK) of a sampled row, when the merge
+      // kernels follow this launch: its k keys are compacted into the row's
+      // candidate slots (cap >= 4k) and sorted there by several workgroups.
+      const bool to_merge = p.merge && ph == PH_FILTER && k > (uint32_t)MERGE_CHUNK;
+      uint64_t* mk = p.cand + p.cand_base[r];
       if (threadIdx.x == 0) slot_cnt = 0;
       if (in_lds) {
         // In place, one chunk of CH keys per thread at a time: every slot
@@ -971,7 +1041,7 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
         }
       } else {
         __syncthreads();
-        constexpr int UN = 4;  // loads in flight per thread
+        constexpr int UN = 8;  // loads in flight per thread
         for (uint32_t c0 = 0; c0 < n; c0 += SEL_NT * UN) {  // uniform trip count
           uint64_t x[UN];
 #pragma unroll
@@ -979,24 +1049,38 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
             const uint32_t i = c0 + threadIdx.x + (uint32_t)u * SEL_NT;
             x[u] = i < n ? ldg(keys + i) : 0ull;
           }
+          // In place (keys == mk): every slot handed out is below the number
+          // of keys read so far; the barrier makes all of this step's reads
+          // precede its writes.
+          if (to_merge) __syncthreads();
 #pragma unroll
           for (int u = 0; u < UN; ++u) {
             const uint32_t i = c0 + threadIdx.x + (uint32_t)u * SEL_NT;
             const bool take = i < n && x[u] >= kth;
             const uint32_t slot = wave_slot(&slot_cnt, take);
-            if (take && slot < k)
-              sbuf[slot] = order_key<DT>(mode, (uint32_t)(x[u] >> 32), ~(uint32_t)x[u]);
+            if (take && slot < k) {
+              const uint64_t o = order_key<DT>(mode, (uint32_t)(x[u] >> 32), ~(uint32_t)x[u]);
+              if (to_merge) mk[slot] = o;
+              else sbuf[slot] = o;
+            }
           }
+          if (to_merge) __syncthreads();
         }
       }
       __syncthreads();
       if (threadIdx.x == 0 && slot_cnt != k) atomicOr(&p.hdr->status, (uint32_t)ST_INTERNAL);
       RK_STAMP(p, r, 11);
-      sort_and_write<DT, LARGEST>(p, d, sbuf, k, mode, r);
+      if (to_merge) {
+        if (in_lds)
+          for (uint32_t i = threadIdx.x; i < k; i += SEL_NT) mk[i] = sbuf[i];
+        next_phase = PH_MERGE;
+      } else {
+        sort_and_write<DT, LARGEST>(p, d, sbuf, k, mode, r);
+      }
       RK_STAMP(p, r, 12);
     }
     __syncthreads();
-    if (threadIdx.x == 0) p.st[r].phase = PH_DONE;
+    if (threadIdx.x == 0) p.st[r].phase = next_phase;
     __syncthreads();
   }
   if (p.publish && threadIdx.x == 0) {
@@ -1014,6 +1098,145 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
 }
 
 // ---------------------------------------------------------------------------
+// Large selections, k > MERGE_CHUNK (launched only when max_k exceeds it).
+// The select leaves such a row's k order-keyed composite keys in its
+// candidate slots (PH_MERGE, a final phase no later kernel acts on).
+// k_merge_sort sorts every chunk of MERGE_CHUNK keys in LDS, one workgroup
+// per chunk, the chunks of all rows at once; k_merge_rank gives each key its
+// output position: its index in its own sorted chunk plus, for every other
+// chunk of the row, the number of that chunk's keys ordered before it (a
+// binary search in LDS; the keys are unique), and writes value and index
+// there.  A 16384-key selection is then four 4096-key sorts and four rank
+// passes on four CUs instead of one 16384-key sort on one.  (The reference
+// sorts K > 4096 with a host loop of per-row Thrust sorts,
+// topk_radixselect.h:285-307.)
+// ---------------------------------------------------------------------------
+// Chunks per row of the merge kernels: rows with k above the LDS select
+// (lds_keys) never reach them (they take the exact passes and ordered emit).
+RK_HOST_DEV int merge_chunks(const KParams& p) {
+  const int k = p.max_k < p.lds_keys ? p.max_k : p.lds_keys;
+  return (k + MERGE_CHUNK - 1) / MERGE_CHUNK;
+}
+
+// Threads of the chunk sort: 4 keys per thread, 4 waves per SIMD (measured
+// on MI355X for 4096 keys: 31 us at 1024 threads, 33 at 512, 53 at 256; the
+// bitonic network's 64-bit compare-exchanges are VALU-bound and one wave per
+// SIMD leaves their latency exposed).
+constexpr int MS_NT = 1024;
+constexpr int MS_E = MERGE_CHUNK / MS_NT;   // keys per thread (registers)
+
+template <int DT>
+__global__ __launch_bounds__(MS_NT) void k_merge_sort(KParams p) {
+  __shared__ uint64_t xs[MERGE_CHUNK];
+  const int cpr = merge_chunks(p);
+  const int64_t items = (int64_t)p.R * cpr;
+  for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
+    const int r = (int)(it / cpr);
+    const uint32_t c0 = (uint32_t)(it % cpr) * MERGE_CHUNK;
+    if (p.st[r].phase != PH_MERGE) continue;  // uniform
+    const uint32_t k = (uint32_t)p.desc[r].k;
+    if (c0 >= k) continue;
+    const uint32_t n = min(k - c0, (uint32_t)MERGE_CHUNK);
+    uint64_t* keys = p.cand + p.cand_base[r] + c0;
+    [[maybe_unused]] const int sb = 16 + 4 * (int)(it % cpr);  // phase-timing slots
+    RK_STAMP(p, r, sb);
+    uint64_t v[MS_E];
+#pragma unroll
+    for (int e = 0; e < MS_E; ++e) {
+      const uint32_t i = threadIdx.x * MS_E + e;
+      v[e] = i < n ? ldg(keys + i) : 0ull;  // zero pads: sorts last
+    }
+    RK_STAMP(p, r, sb + 1);
+    bitonic_sort_regs<MS_NT, MS_E>(v, xs);
+    RK_STAMP(p, r, sb + 2);
+#pragma unroll
+    for (int e = 0; e < MS_E; ++e) {
+      const uint32_t i = threadIdx.x * MS_E + e;
+      if (i < n) keys[i] = v[e];
+    }
+    __syncthreads();
+    RK_STAMP(p, r, sb + 3);
+  }
+}
+
+template <int DT, bool LARGEST>
+__global__ __launch_bounds__(SEL_NT) void k_merge_rank(KParams p) {
+  using U = typename Traits<DT>::U;
+  constexpr int PER = MERGE_CHUNK / SEL_NT;  // own keys per thread
+  extern __shared__ uint64_t sbuf[];         // all k keys of the row
+  const int mode = order_mode(p.flags);
+  const int cpr = merge_chunks(p);
+  const int64_t items = (int64_t)p.R * cpr;
+  for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
+    const int r = (int)(it / cpr);
+    const int c = (int)(it % cpr);
+    const uint32_t c0 = (uint32_t)c * MERGE_CHUNK;
+    if (p.st[r].phase != PH_MERGE) continue;  // uniform
+    const RowDesc d = p.desc[r];
+    const uint32_t k = (uint32_t)d.k;
+    if (c0 >= k) continue;
+    const uint32_t n = min(k - c0, (uint32_t)MERGE_CHUNK);
+    [[maybe_unused]] const int sb = 32 + 4 * c;  // phase-timing slots
+    RK_STAMP(p, r, sb);
+    load_to_lds<SEL_NT, 8>(sbuf, p.cand + p.cand_base[r], k, [](uint64_t x, uint32_t) { return x; });
+    __syncthreads();
+    RK_STAMP(p, r, sb + 1);
+    uint64_t x[PER];
+    uint32_t rank[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const uint32_t i = threadIdx.x + (uint32_t)j * SEL_NT;
+      x[j] = i < n ? sbuf[c0 + i] : 0ull;
+      rank[j] = i;
+    }
+    for (int cc = 0; cc < cpr; ++cc) {
+      const uint32_t d0 = (uint32_t)cc * MERGE_CHUNK;
+      if (cc == c || d0 >= k) continue;  // uniform
+      const uint32_t m = min(k - d0, (uint32_t)MERGE_CHUNK);
+      const uint64_t* other = sbuf + d0;
+      // keys of `other` before x: a prefix (sorted descending), found by a
+      // branch-free binary search, the PER searches of a thread interleaved
+      uint32_t cnt[PER];
+#pragma unroll
+      for (int j = 0; j < PER; ++j) cnt[j] = 0;
+      for (uint32_t step = next_pow2(m + 1) >> 1; step > 0; step >>= 1) {
+        uint64_t o[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) o[j] = other[min(cnt[j] + step, m) - 1];  // unconditional
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+          const uint32_t q = cnt[j] + step;
+          if (q <= m && o[j] > x[j]) cnt[j] = q;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < PER; ++j) rank[j] += cnt[j];
+    }
+    RK_STAMP(p, r, sb + 2);
+    const U* rb = static_cast<const U*>(d.base);
+    U* ov = static_cast<U*>(d.out_v);
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const uint32_t i = threadIdx.x + (uint32_t)j * SEL_NT;
+      if (i >= n) continue;
+      uint32_t sk, pos;
+      order_decode<DT>(mode, x[j], sk, pos);
+      const uint32_t bits = sk ? key_to_bits<DT, LARGEST>(sk) : (uint32_t)ldg(rb + pos);
+      ov[rank[j]] = (U)bits;
+      d.out_i[rank[j]] = d.idx_in ? d.idx_in[pos] : (int32_t)pos;
+    }
+    if (c == 0) {
+      for (int64_t i = (int64_t)k + threadIdx.x; i < d.slots; i += SEL_NT) {
+        ov[i] = (U)0;
+        d.out_i[i] = -1;
+      }
+    }
+    __syncthreads();
+    RK_STAMP(p, r, sb + 3);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Exact passes (RadiK's countBin / selectBin, radixselect_l.cuh:14-126,
 // 173-267, on the selection key): per pass, a histogram of the next digit of
 // the keys that share the resolved prefix, then the bin holding the k-th
@@ -1021,6 +1244,8 @@ __global__ __launch_bounds__(SEL_NT) void k_select_finalize(KParams p) {
 // fit its candidate slots (threshold known exactly), or, with every bit
 // resolved, as PH_ORDERED.
 // ---------------------------------------------------------------------------
+// Exact calls (RK_EXACT) only: in sampled calls the sample and select
+// kernels start the rows they send to the exact passes (exact_start_row).
 __global__ __launch_bounds__(256) void k_exact_init(KParams p) {
   if (p.hdr->fallback_rows == 0) return;
   for (int r = blockIdx.x; r < p.R; r += gridDim.x) {
@@ -1183,7 +1408,8 @@ __global__ __launch_bounds__(256) void k_hist_select(KParams p, int key_bits) {
 // ---------------------------------------------------------------------------
 // Ordered emit for rows whose ties at the k-th key V exceed the candidate
 // slots (e.g. an all-equal row).  Per tile, counts of keys > V and == V;
-// a per-row scan over tiles; then each tile writes, in row-position order,
+// then each tile writes (its prefix summed over the row's earlier tiles), in
+// row-position order,
 // its keys > V and those == V whose rank among equals is below rem.  The
 // output position of a selected element is gt_before + min(rem, eq_before).
 // ---------------------------------------------------------------------------
@@ -1254,26 +1480,6 @@ __global__ __launch_bounds__(STREAM_NT) void k_emit_count(KParams p) {
   });
 }
 
-__global__ __launch_bounds__(1024) void k_emit_scan(KParams p) {
-  if (p.hdr->ordered_rows == 0) return;
-  __shared__ uint32_t scratch[16];
-  for (int r = blockIdx.x; r < p.R; r += gridDim.x) {
-    if (p.st[r].phase != PH_ORDERED) continue;
-    const uint32_t t0 = p.tile_start[r], t1 = p.tile_start[r + 1];
-    uint32_t cg = 0, ce = 0;
-    for (uint32_t base = t0; base < t1; base += 1024) {
-      const uint32_t t = base + threadIdx.x;
-      const uint2 c = t < t1 ? p.tile_cnt[t] : make_uint2(0u, 0u);
-      uint32_t tg, te;
-      const uint32_t eg = block_excl_scan<1024>(c.x, scratch, tg);
-      const uint32_t ee = block_excl_scan<1024>(c.y, scratch, te);
-      if (t < t1) p.tile_pre[t] = make_uint2(cg + eg, ce + ee);
-      cg += tg;
-      ce += te;
-    }
-  }
-}
-
 template <int DT, bool LARGEST>
 __global__ __launch_bounds__(STREAM_NT) void k_emit_write(KParams p) {
   if (p.hdr->ordered_rows == 0) return;
@@ -1289,9 +1495,18 @@ __global__ __launch_bounds__(STREAM_NT) void k_emit_write(KParams p) {
     const UT* rb = static_cast<const UT*>(d.base);
     UT* ov = static_cast<UT*>(d.out_v);
     int32_t* oi = d.out_i;
+    // Keys > V and == V in the row's tiles before this segment (the counts
+    // k_emit_count wrote); within the segment the running totals carry on.
+    const uint2* tc = p.tile_cnt + p.tile_start[r];
+    uint32_t pg = 0, pe = 0;
+    for (uint32_t t = threadIdx.x; t < ta; t += STREAM_NT) {
+      const uint2 c = tc[t];
+      pg += c.x;
+      pe += c.y;
+    }
+    uint32_t run_g = block_sum<STREAM_NT>(pg, scratch);
+    uint32_t run_e = block_sum<STREAM_NT>(pe, scratch);
     for (uint32_t t = ta; t < tb; ++t) {
-      const uint2 pre = p.tile_pre[p.tile_start[r] + t];
-      uint32_t run_g = pre.x, run_e = pre.y;
       const int64_t es = (int64_t)t * p.tune.tile;
       int64_t ee = es + p.tune.tile;
       if (ee > d.len) ee = d.len;
@@ -1400,6 +1615,8 @@ static void set_lds_attrs(size_t dyn) {
                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_finalize_ordered<DT, LARGEST>),
                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_merge_rank<DT, LARGEST>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
 }
 
 // Optional timing of the streaming filter kernel (the dominant kernel):
@@ -1459,6 +1676,22 @@ static int resident_grid(K kernel, int threads, size_t dyn = 0) {
   return g;
 }
 
+// Compute units of the current device (memoized per device).
+static int device_cus() {
+  static std::mutex mu;
+  static std::vector<int> memo;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int)memo.size() <= dev) memo.resize(dev + 1, 0);
+  if (memo[dev] <= 0) {
+    int cus = 0;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    memo[dev] = cus > 0 ? cus : 1;
+  }
+  return memo[dev];
+}
+
 template <int DT, bool LARGEST, int U>
 static hipError_t run_typed_u(const KParams& p, const Launch& L0) {
   constexpr int KB = Traits<DT>::KEY_BITS;
@@ -1474,11 +1707,26 @@ static hipError_t run_typed_u(const KParams& p, const Launch& L0) {
   const int g_sample = rows(resident_grid(k_sample<DT, LARGEST>, SEL_NT, dyn_sample));
   const int g_select = rows(resident_grid(k_select_finalize<DT, LARGEST>, SEL_NT, dyn));
   const int g_init = rows(resident_grid(k_exact_init, 256));
-  const int g_hsel = rows(resident_grid(k_hist_select, 256));
-  const int g_escan = rows(resident_grid(k_emit_scan, 1024));
-  const int g_ford = rows(resident_grid(k_finalize_ordered<DT, LARGEST>, SEL_NT, dyn));
+  int g_hsel = rows(resident_grid(k_hist_select, 256));
+  int g_ford = rows(resident_grid(k_finalize_ordered<DT, LARGEST>, SEL_NT, dyn));
   KParams p1 = p;  // fallback round: kernels return at once when no row fell back
   p1.round = 1;
+  // In a sampled call the fallback round is almost always idle, and an idle
+  // launch costs about as much as dispatching its workgroups: one workgroup
+  // per CU each (measured: 13 idle launches of full persistent grids took
+  // ~4.5 us each on 512-row calls, ~1.5 us at one workgroup per CU).  Rows
+  // that do fall back still stream at 256 workgroups.  Exact calls keep the
+  // full grids (their passes are the work).
+  const bool exact_call = (p.flags & FL_EXACT) != 0;
+  int fb_f = fgrid, fb_h = hgrid, fb_sel = g_select;
+  if (!exact_call) {
+    const int cus = device_cus();
+    fb_f = std::min(fb_f, cus);
+    fb_h = std::min(fb_h, cus);
+    fb_sel = std::min(fb_sel, cus);
+    g_hsel = std::min(g_hsel, cus);
+    g_ford = std::min(g_ford, cus);
+  }
   if (L0.part != PART_FALLBACK) {
     if (p.R <= 1024) {
       k_plan_scan<DT, true><<<1, 1024, 0, s>>>(p);
@@ -1495,22 +1743,34 @@ static hipError_t run_typed_u(const KParams& p, const Launch& L0) {
     if (timed) timing_end(s, ev);
     KParams pm = p;
     pm.publish = L0.part == PART_MAIN ? L0.publish : nullptr;
+    pm.merge = merge_chunks(p) > 1;
     k_select_finalize<DT, LARGEST><<<g_select, SEL_NT, dyn, s>>>(pm);
+    if (pm.merge) {
+      const int cpr = merge_chunks(p);
+      const int64_t items = (int64_t)p.R * cpr;
+      const size_t drank = (size_t)cpr * MERGE_CHUNK * sizeof(uint64_t);
+      const int g_ms = (int)std::min<int64_t>(items, resident_grid(k_merge_sort<DT>, MS_NT));
+      const int g_mr = (int)std::min<int64_t>(
+          items, resident_grid(k_merge_rank<DT, LARGEST>, SEL_NT, drank));
+      k_merge_sort<DT><<<g_ms, MS_NT, 0, s>>>(pm);
+      k_merge_rank<DT, LARGEST><<<g_mr, SEL_NT, drank, s>>>(pm);
+    }
   }
   if (L0.part == PART_MAIN) return hipGetLastError();
-  k_exact_init<<<g_init, 256, 0, s>>>(p1);
+  // rows entering the exact passes are initialised by the kernel that sends
+  // them there; only an exact call starts them all here
+  if (exact_call) k_exact_init<<<g_init, 256, 0, s>>>(p1);
   const int passes = (KB + p.tune.hist_bits - 1) / p.tune.hist_bits;
   for (int i = 0; i < passes; ++i) {
-    k_hist<DT, LARGEST, U><<<hgrid, STREAM_NT, 0, s>>>(p1);
+    k_hist<DT, LARGEST, U><<<fb_h, STREAM_NT, 0, s>>>(p1);
     k_hist_select<<<g_hsel, 256, 0, s>>>(p1, KB);
   }
-  k_filter<DT, LARGEST, U><<<fgrid, STREAM_NT, 0, s>>>(p1);
+  k_filter<DT, LARGEST, U><<<fb_f, STREAM_NT, 0, s>>>(p1);
   // COPYALL / SMALL rows of an exact-path call are finalized here too.
-  KParams pf = (p.flags & FL_EXACT) ? p : p1;
-  k_select_finalize<DT, LARGEST><<<g_select, SEL_NT, dyn, s>>>(pf);
-  k_emit_count<DT, LARGEST><<<fgrid, STREAM_NT, 0, s>>>(p1);
-  k_emit_scan<<<g_escan, 1024, 0, s>>>(p1);
-  k_emit_write<DT, LARGEST><<<fgrid, STREAM_NT, 0, s>>>(p1);
+  KParams pf = exact_call ? p : p1;
+  k_select_finalize<DT, LARGEST><<<fb_sel, SEL_NT, dyn, s>>>(pf);
+  k_emit_count<DT, LARGEST><<<fb_f, STREAM_NT, 0, s>>>(p1);
+  k_emit_write<DT, LARGEST><<<fb_f, STREAM_NT, 0, s>>>(p1);
   k_finalize_ordered<DT, LARGEST><<<g_ford, SEL_NT, dyn, s>>>(p1);
   return hipGetLastError();
 }

@@ -18,7 +18,12 @@ enum Phase : uint32_t {
   PH_HIST = 5,     // exact radix passes over the row (fallback / plain path)
   PH_ORDERED = 6,  // k-th key V known but its ties overflow the candidate
                    // buffer: ordered emit in row-position order
+  PH_MERGE = 7,    // k > MERGE_CHUNK selected keys (order-keyed) in the row's
+                   // candidate slots: sorted in chunks, then ranked and written
 };
+
+// Keys per chunk of the multi-workgroup sort of large selections.
+constexpr int MERGE_CHUNK = 4096;
 
 enum RowFlag : uint32_t {
   RF_NEED_INIT = 1u,  // PH_HIST row whose histogram/state must be reset
@@ -114,8 +119,7 @@ struct KParams {
   uint32_t* order;       // [2R] rows by descending cost class: largest k first
                          // (select), then longest first (sample)
   uint32_t* ghist;       // [R][2048]
-  uint2* tile_cnt;       // [tile_capacity]
-  uint2* tile_pre;       // [tile_capacity]
+  uint2* tile_cnt;       // [tile_capacity] ordered emit: keys > V, == V per tile
   uint32_t* chunk_row;   // [tile_capacity] row of each filter chunk, or ~0u
   uint64_t* cand;        // [cand_capacity]
   uint64_t tile_capacity;
@@ -125,6 +129,8 @@ struct KParams {
   int32_t sample_keys;  // LDS key capacity of the sample kernel (this launch)
   int32_t round;        // 0: sampled round; 1: fallback round (skips when idle)
   int32_t chunk_tiles;  // tiles per chunk of the interleaved filter partition
+  int32_t merge;        // select: sort rows with k > MERGE_CHUNK in chunks
+                        // (the merge kernels follow this select launch)
   Tune tune;
   // When set (sampled-round select only): the last select workgroup to finish
   // copies the header's first 8 words here (mapped host memory).

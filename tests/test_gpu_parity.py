@@ -397,3 +397,28 @@ def test_workload_c1_full_size(flags, exact):
     bits = from_device_bits(vals, "f32")[: w.total]
     st = _check("f32", bits, w.offsets(), w.ks, flags | (EXACT if exact else 0))
     assert st[0] == 0
+
+
+@pytest.mark.parametrize("dt", ["f32", "bf16", "f16"])
+@pytest.mark.parametrize("flags", FLAG_SETS)
+def test_large_k_chunked_sort(dt, flags):
+    """Sampled rows with k above one sort chunk (4096 keys) are sorted in
+    chunks by several workgroups and ranked across chunks: k at and around
+    the chunk multiples, duplicate-heavy rows (ties broken by position),
+    caller indices, mixed with small-k and LDS-direct rows in one call."""
+    rng = np.random.default_rng(301 + DT_CODE[dt] * 7 + flags)
+    lens = np.array([200_000, 300_001, 150_000, 1 << 20, 90_000, 600_000, 12_000, 70_000],
+                    np.int64)
+    ks = np.array([4097, 8192, 8193, 16384, 5000, 12_001, 6000, 100], np.int32)
+    off = np.zeros(len(lens) + 1, np.int64)
+    off[1:] = np.cumsum(lens)
+    bits = _gen_bits(dt, 1, int(off[-1]), 303 + flags)
+    # ties inside the selection that span chunks: 3000 copies of 2.5 in row 2
+    # (about the 0.6 % quantile of N(0,1); the k-th value is near 1.6) and
+    # 5000 copies of 2.2 in row 5 (its k-th value is near 2.05)
+    for r, v, cnt in ((2, 2.5, 3000), (5, 2.2, 5000)):
+        pos = int(off[r]) + rng.choice(int(lens[r]), cnt, replace=False)
+        bits[pos] = _bits_of(dt, np.full(cnt, v, np.float32))
+    idx_in = rng.integers(-2**31, 2**31 - 1, int(off[-1])).astype(np.int32)
+    _check(dt, bits, off, ks, flags)
+    _check(dt, bits, off, ks, flags, idx_in=idx_in)
