@@ -41,18 +41,39 @@ def log(msg: str) -> None:
     sys.stderr.flush()
 
 
-def profiled_traffic(workload: str):
-    """HBM bytes per k_filter launch measured by the rocprofv3 FETCH_SIZE /
-    WRITE_SIZE passes summarized in profiles/k_filter_<workload>.txt (scaled
-    by known-size kernels), or None when no summary exists."""
+PROFILE_KEY = {"c4_batched_uneven_f32": "c4_f32", "c4_batched_uneven_bf16": "c4_bf16",
+               "c4_batched_uneven_f16": "c4_f16"}
+
+
+def profiled_traffic(workload: str) -> dict:
+    """HBM bytes per k_filter launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+    passes summarized in profiles/k_filter_<workload>.txt, with what that
+    figure rests on: the profiled run's tunables and the scale factors that
+    turn the counters into bytes (each calibrated on a kernel of known size:
+    reads on the read probe, writes on the input generator), so the figure is
+    relative to those kernels.  This run's own tunables may differ (the
+    autotuner's winner varies from run to run within ~1 % of the step)."""
     path = os.path.join(ROOT, "profiles", f"k_filter_{workload}.txt")
+    out = {"traffic": None}
     if not os.path.exists(path):
-        return None
+        return out
+    src = {"profile": os.path.relpath(path, ROOT)}
     with open(path) as f:
         for line in f:
             if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":", 1)[1])
-    return None
+                out["traffic"] = float(line.split(":", 1)[1])
+            elif line.startswith("# tunables"):
+                src["profiled_tunables"] = line.split(":", 1)[1].strip()
+            elif "read scale" in line and "=" in line:
+                src["read_scale"] = float(line.rsplit("=", 1)[1])
+            elif "write scale" in line and "=" in line:
+                src["write_scale"] = float(line.rsplit("=", 1)[1])
+            elif line.startswith("traffic_over_algorithmic:"):
+                src["traffic_over_algorithmic"] = float(line.split(":", 1)[1])
+    src["calibration"] = ("FETCH_SIZE scaled by the read probe's bytes / its FETCH_SIZE, "
+                          "WRITE_SIZE by the generator's bytes / its WRITE_SIZE")
+    out["traffic_source"] = src
+    return out
 
 
 def make_workload(name: str):
@@ -90,8 +111,17 @@ def main() -> int:
                     help="requests in the serving run (0: skip it)")
     ap.add_argument("--serve-load", type=float, default=0.7,
                     help="offered rate as a fraction of the plain server's capacity")
+    ap.add_argument("--serve-hi-load", type=float, default=0.5,
+                    help="second offered rate (batched server only), as a fraction of the "
+                         "batched server's capacity")
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the serving measurement (prints its JSON object)")
+    ap.add_argument("--no-bf16", action="store_true",
+                    help="default workload: skip the bf16 run of the same batch")
+    ap.add_argument("--no-sweeps", dest="sweeps", action="store_false",
+                    help="skip the single-row and quantile sweeps")
+    ap.add_argument("--sweeps-only", action="store_true",
+                    help="run only the sweeps (prints their JSON object)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -114,7 +144,46 @@ def main() -> int:
     if args.workload == "knn":
         print(json.dumps({"knn": knn_bench(args)}), flush=True)
         return 0
+    if args.sweeps_only:
+        print(json.dumps({"sweeps": sweeps(args)}), flush=True)
+        return 0
     w = make_workload(args.workload)
+    result = measure(w, args, primary=True)
+    if result.get("bit_exact") is False:
+        print(json.dumps(result), flush=True)
+        return 1
+    torch.cuda.empty_cache()
+    if args.workload == "c4_f32" and not args.no_bf16:
+        # the same batch in bf16 (rounded from the same fp32 values), on the
+        # same clock: its own roofline, plain path, CPU baseline and check
+        extra = measure(make_workload("c4_bf16"), args, primary=False)
+        result["c4_bf16"] = extra
+        torch.cuda.empty_cache()
+        if extra.get("bit_exact") is False:
+            print(json.dumps(result), flush=True)
+            return 1
+    if args.sweeps:
+        result["sweeps"] = sweeps(args)
+        torch.cuda.empty_cache()
+
+    if args.serve_requests > 0:
+        result["serving"] = serving(args)
+
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def measure(w, args, primary: bool) -> dict:
+    """One batched workload at full size: autotune, eager and graph-replay
+    timing, the filter kernel's roofline, the read probe, the plain per-row
+    path, the CPU baseline and the bit-exact check of the timed output.
+    ``primary``: the fields of the top-level bench line (metric, value, ...)."""
+    import numpy as np
+    import torch
+
+    import radik_amd as rk
+    import radik_amd.workloads as W
+
     es = 4 if w.dtype == "f32" else 2
     dt = {"f32": rk.RK_F32, "bf16": rk.RK_BF16, "f16": rk.RK_F16}[w.dtype]
     log(f"workload {w.name}: {w.rows} rows, {w.total} keys, max k {w.max_k}")
@@ -255,7 +324,7 @@ def main() -> int:
                    "largest": True, "sorted": True, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "k_filter", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(args.workload),
+                     **profiled_traffic(PROFILE_KEY.get(w.name, w.name)),
                      "algorithmic_bytes_per_launch": filt_bytes,
                      "avg_launch_ms": filt_avg_ms, "launches_timed": filt_n},
         "pipeline": {"algorithmic_bytes": alg_bytes, "achieved_gbs": pipeline_gbs,
@@ -345,8 +414,7 @@ def main() -> int:
                                        "against": "oracle (CPU restatement), values and indices"}
         if not exact:
             log("timed output differs from the oracle")
-            print(json.dumps(result), flush=True)
-            return 1
+            return result
         result["cpu_baseline"] = {
             "value": float(s_off[-1]) / c_s, "unit": "keys/s", "cores": nth, "kind": "port",
             "cpu_model": cpu_model(), "host_threads": orc.hardware_threads(),
@@ -356,16 +424,155 @@ def main() -> int:
                       + f" ({len(rows)} rows, {int(s_off[-1])} keys), std::nth_element + "
                       f"std::sort per row, {nth} threads, {c_s:.2f} s"}
         log(f"cpu baseline: {float(s_off[-1]) / c_s / 1e9:.3f} Gkeys/s on {nth} threads")
+    if not primary:
+        for key in ("metric", "n_gpus", "steps", "warmup", "higher_is_better", "scaling",
+                    "vs_baseline"):
+            result.pop(key, None)
+    del vals, ws, out_v, out_i
+    return result
 
-    if args.serve_requests > 0:
-        del vals, ws, out_v, out_i
-        if not args.no_plain and args.plain_steps > 0:
-            del pws
-        torch.cuda.empty_cache()
-        result["serving"] = serving(args)
 
-    print(json.dumps(result), flush=True)
-    return 0
+def sweeps(args) -> dict:
+    """The reference's own benchmark sweeps, timed on this path (values
+    U[0,1) fp32 from a seed, generated on the device; every point is one call
+    captured in a hipGraph and replayed, default tunables):
+
+      single_query  one row of N = 2^21 .. 2^29 keys, K = 2^4 .. 2^12, largest,
+                    sorted (eval/1-simple-topk.py:18-27); the sampled path and
+                    the exact multi-pass path (RK_EXACT), and the streaming
+                    filter's own bandwidth (HIP events around it);
+      quantile      16 rows of N = 2^16 .. 2^22 keys, K = 512, N/100, N/4 and
+                    N/2, unsorted (eval/ex1-median.py:11-26).
+
+    Roofline: algorithmic bytes (one read of every key, K x (value + index)
+    written, 16 B per row) / time / 8 TB/s."""
+    import numpy as np
+    import torch
+
+    import radik_amd as rk
+
+    L = rk.lib()
+    gstream = torch.cuda.Stream()
+    sp = gstream.cuda_stream
+    reps = 10
+
+    def replay_ms(call):
+        torch.cuda.synchronize()
+        with rk.Graph.capture(gstream) as g:
+            call()
+        for _ in range(2):
+            g.launch(gstream)
+        gstream.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(gstream)
+        for _ in range(reps):
+            g.launch(gstream)
+        e1.record(gstream)
+        gstream.synchronize()
+        g.close()
+        return e0.elapsed_time(e1) / reps
+
+    nmax, kmax = 1 << 29, 1 << 12
+    vals = torch.empty(nmax, dtype=torch.float32, device="cuda")
+    rk.gen_fill(vals, rk.GEN_UNIFORM, 11, 0)
+    flags = rk.RK_LARGEST | rk.RK_SORTED
+    ws_bytes = max(rk.workspace_size(rk.RK_F32, 1, nmax, kmax, flags),
+                   rk.workspace_size(rk.RK_F32, 1, nmax, kmax, flags | rk.RK_EXACT))
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+    ov = torch.empty(kmax, dtype=torch.float32, device="cuda")
+    oi = torch.empty(kmax, dtype=torch.int32, device="cuda")
+    t0 = time.perf_counter()
+    single = []
+    for ln in range(21, 30):
+        n = 1 << ln
+        d_off = torch.tensor([0, n], dtype=torch.int64, device="cuda")
+        for ko in range(4, 13):
+            k = 1 << ko
+            d_k = torch.tensor([k], dtype=torch.int32, device="cuda")
+            d_oo = torch.tensor([0, k], dtype=torch.int64, device="cuda")
+
+            def call(fl):
+                rc = L.rk_topk_batched(rk.RK_F32, vals.data_ptr(), d_off.data_ptr(),
+                                       d_k.data_ptr(), 1, n, k, fl, None, ov.data_ptr(),
+                                       oi.data_ptr(), d_oo.data_ptr(), ws.data_ptr(), ws_bytes, sp)
+                if rc != 0:
+                    raise rk.RadikError(rc, "rk_topk_batched (sweep)")
+
+            ms = replay_ms(lambda: call(flags))
+            st = rk.status(ws, gstream)
+            # the filter alone, eager, with HIP events around it
+            rk.timing_read()
+            rk.timing_enable(True)
+            for _ in range(3):
+                call(flags)
+            gstream.synchronize()
+            rk.timing_enable(False)
+            f_ms, f_n = rk.timing_read()
+            ems = replay_ms(lambda: call(flags | rk.RK_EXACT))
+            est = rk.status(ws, gstream)
+            if st[0] or est[0]:
+                raise RuntimeError(f"sweep N=2^{ln} K={k}: device status {st} / {est}")
+            alg = n * 4 + k * 8 + 16
+            single.append([n, k, round(ms, 5), n / (ms / 1e3), round(alg / (ms / 1e3) / 1e9 /
+                           HBM_PEAK_GBS, 4), round(ems, 5),
+                           round(n * 4 / (f_ms / max(f_n, 1) / 1e3) / 1e9, 1) if f_n else None])
+    big = [r for r in single if r[0] == nmax]
+    log(f"sweep single_query: {len(single)} points in {time.perf_counter() - t0:.1f} s; "
+        f"N=2^29: {min(r[2] for r in big):.3f}-{max(r[2] for r in big):.3f} ms, "
+        f"filter {max(r[6] for r in big):.0f} GB/s")
+    del ws
+    # quantile: 16 rows, unsorted, K up to N/2 (exact passes + ordered emit above the
+    # LDS select)
+    batch = 16
+    qflags = rk.RK_LARGEST
+    nq_max = 1 << 22
+    q_ws_bytes = rk.workspace_size(rk.RK_F32, batch, batch * nq_max, nq_max // 2, qflags)
+    qws = torch.empty(q_ws_bytes, dtype=torch.uint8, device="cuda")
+    qov = torch.empty(batch * (nq_max // 2), dtype=torch.float32, device="cuda")
+    qoi = torch.empty(batch * (nq_max // 2), dtype=torch.int32, device="cuda")
+    t0 = time.perf_counter()
+    quant = []
+    for ln in range(16, 23):
+        n = 1 << ln
+        d_off = torch.arange(batch + 1, dtype=torch.int64, device="cuda") * n
+        for k in (512, n // 100, n // 4, n // 2):
+            d_k = torch.full((batch,), k, dtype=torch.int32, device="cuda")
+            d_oo = torch.arange(batch + 1, dtype=torch.int64, device="cuda") * k
+
+            def qcall():
+                rc = L.rk_topk_batched(rk.RK_F32, vals.data_ptr(), d_off.data_ptr(),
+                                       d_k.data_ptr(), batch, batch * n, k, qflags, None,
+                                       qov.data_ptr(), qoi.data_ptr(), d_oo.data_ptr(),
+                                       qws.data_ptr(), q_ws_bytes, sp)
+                if rc != 0:
+                    raise rk.RadikError(rc, "rk_topk_batched (quantile sweep)")
+
+            ms = replay_ms(qcall)
+            st = rk.status(qws, gstream)
+            if st[0]:
+                raise RuntimeError(f"quantile sweep N=2^{ln} K={k}: device status {st}")
+            alg = batch * (n * 4 + k * 8 + 16)
+            quant.append([n, k, round(ms, 5), batch * n / (ms / 1e3),
+                          round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4), st[1], st[2]])
+    log(f"sweep quantile: {len(quant)} points in {time.perf_counter() - t0:.1f} s")
+    del vals, qws, qov, qoi
+    return {
+        "single_query": {
+            "source": "radik eval/1-simple-topk.py:18-27 (batch 1)",
+            "values": "U[0,1) fp32, seed 11, on the device", "largest": True, "sorted": True,
+            "tunables": "defaults", "timing": "hipGraph replays, mean of 10",
+            "columns": ["N", "K", "ms", "keys_per_s", "roofline_frac", "exact_path_ms",
+                        "filter_gbs"],
+            "points": single},
+        "quantile": {
+            "source": "radik eval/ex1-median.py:11-26 (batch 16, unsorted)",
+            "values": "U[0,1) fp32, seed 11, on the device", "largest": True, "sorted": False,
+            "tunables": "defaults", "timing": "hipGraph replays, mean of 10",
+            "columns": ["N", "K", "ms", "keys_per_s", "roofline_frac", "exact_rows",
+                        "ordered_rows"],
+            "points": quant},
+    }
 
 
 def knn_bench(args) -> dict:
@@ -475,6 +682,44 @@ def percentile(x, q):
     return float(x[max(1, int(np.ceil(q * len(x)))) - 1]) if len(x) else 0.0
 
 
+def verify_requests(bodies, body, body_len, lens, ks, ov, oi, oo, every: int = 100) -> dict:
+    """Checks every `every`-th request's output (values and indices, in the
+    slots the replay wrote) against the oracle on the host; raises if any
+    differs."""
+    import numpy as np
+
+    from oracle import oracle as orc
+    sel = np.arange(0, len(lens), every)
+    flat = bodies.view(-1).view(torch_int32())
+    s_off = np.zeros(len(sel) + 1, np.int64)
+    s_off[1:] = np.cumsum(lens[sel])
+    host = np.empty(int(s_off[-1]), np.uint32)
+    for j, i in enumerate(sel):
+        b0 = int(body[i]) * body_len
+        host[s_off[j]:s_off[j + 1]] = flat[b0:b0 + int(lens[i])].cpu().numpy().view(np.uint32)
+    # each request has k slots (results then the tail fill), min(len, k) results
+    eb, ei = orc.topk(host, s_off, ks[sel], flags=orc.LARGEST | orc.SORTED,
+                      nthreads=orc.usable_threads())
+    m = np.minimum(lens[sel], ks[sel].astype(np.int64))
+    e_off = np.zeros(len(sel) + 1, np.int64)
+    e_off[1:] = np.cumsum(m)
+    gv = ov.view(torch_int32()).cpu().numpy().view(np.uint32)
+    gi = oi.cpu().numpy()
+    for j, i in enumerate(sel):
+        a = int(oo[i])
+        mm = int(m[j])
+        if not (np.array_equal(gv[a:a + mm], eb[e_off[j]:e_off[j + 1]])
+                and np.array_equal(gi[a:a + mm], ei[e_off[j]:e_off[j + 1]])):
+            raise RuntimeError(f"serving: request {i} differs from the oracle")
+    return {"requests": int(len(sel)), "every": every, "bit_exact": True,
+            "against": "oracle (CPU restatement), values and indices"}
+
+
+def torch_int32():
+    import torch
+    return torch.int32
+
+
 def serving(args) -> dict:
     """Serving-loop measurement.  Request stream: n requests with length
     floor(2^U(12,22)), k floor(2^U(0,12)) and one of four value families
@@ -556,6 +801,28 @@ def serving(args) -> dict:
     out["batched"]["capacity_req_s"] = cap_batched
     out["plain"]["capacity_req_s"] = cap_plain
     out["p99_ratio_plain_over_batched"] = out["plain"]["p99_us"] / out["batched"]["p99_us"]
+
+    # Every 100th request of the last replay (the plain server's, which wrote
+    # the same slots) against the oracle on the host.
+    out["plain"]["verified"] = verify_requests(bodies, body, body_len, lens, ks, ov, oi, oo)
+
+    # A load where batches form: 0.5 x the batched server's own capacity (the
+    # plain server cannot follow at that rate, so it is not replayed there).
+    hi_rate = args.serve_hi_load * cap_batched
+    hi_arrivals = W.poisson_arrivals(n, hi_rate, seed=6)
+    lat, makespan, st = run(False, idx, hi_arrivals)
+    out["batched_high_load"] = {
+        "offered_rate_per_s": hi_rate, "load_vs_batched_capacity": args.serve_hi_load,
+        "p50_us": percentile(lat, 0.50), "p99_us": percentile(lat, 0.99),
+        "mean_us": float(lat.mean()), "max_us": float(lat.max()),
+        "throughput_req_s": n / makespan, "keys_per_s": float(lens.sum()) / makespan,
+        "batches": st["batches"], "mean_batch_rows": st["mean_batch_rows"],
+        "max_batch_rows": st["max_batch_rows"], "fallback_batches": st["fallback_batches"],
+        "peak_inflight_bytes": st["peak_inflight_bytes"],
+        "verified": verify_requests(bodies, body, body_len, lens, ks, ov, oi, oo)}
+    log(f"batched at {hi_rate:.0f} req/s: p50 {out['batched_high_load']['p50_us']:.0f} us, "
+        f"p99 {out['batched_high_load']['p99_us']:.0f} us, mean batch "
+        f"{st['mean_batch_rows']:.1f}, verified {out['batched_high_load']['verified']}")
     del bodies, ov, oi
     torch.cuda.empty_cache()
     return out

@@ -88,53 +88,35 @@ RK_DEV bool bits_is_nan(uint32_t u) {
   else return (u & 0x7fffu) > 0x7f80u;
 }
 
-// Signed-order image of a bit pattern: as a signed integer of the element's
-// width, ord(u) orders exactly as the flip does (ord(u) = flip(u) with the top
-// bit toggled): -NaN < -inf < ... < -0 < +0 < ... < +inf < +NaN.  Integer
-// only, so it does not depend on the float unit's NaN or denormal handling
-// (a signalling NaN stays an ordinary bit pattern).  ord32 maps one f32
-// word; ord16x2 maps the two 16-bit elements of a word independently.
-RK_DEV int32_t ord32(uint32_t u) {
-  return (int32_t)(u ^ ((uint32_t)((int32_t)u >> 31) >> 1));
-}
-RK_DEV uint32_t ord16x2(uint32_t w) {
-  return w ^ (((w >> 15) & 0x00010001u) * 0x7fffu);
+// Prefilter of the streaming filter.  For a key threshold T > 0 the XOR mask
+// m below, chosen by the half of key space T lies in, makes (raw ^ m) as an
+// unsigned integer equal the selection key of every element in that half;
+// the elements of the other half all map below T (none can pass) or all
+// above it (the exact test decides).  So a 16-byte vector can hold a key >= T
+// only if max_u(raw_i ^ m) >= T: one xor and one unsigned max per word
+// (packed halves for 16-bit types).  Integer only, so it does not depend on
+// the float unit's NaN or denormal handling; NaN patterns may pass (their
+// key is 0 < T) and the exact per-element test sel_key(u) >= T removes them.
+// (Checked exhaustively over all 16-bit patterns: no element with a key >= T
+// fails the prefilter.)
+template <int DT, bool LARGEST>
+RK_DEV uint32_t prefilter_mask(uint32_t T) {
+  constexpr uint32_t KM = Traits<DT>::KEY_MASK, TOP = (KM >> 1) + 1;
+  uint32_t m = LARGEST ? (T >= TOP ? TOP : KM) : (T >= TOP ? 0u : (KM ^ TOP));
+  if constexpr (Traits<DT>::KEY_BITS == 16) m |= m << 16;
+  return m;
 }
 
-// Threshold of the vector prefilter in ord space for a key threshold T > 0:
-// for every non-NaN element u,
-//   sel_key(u) >= T  <=>  (LARGEST ? ord(u) >= t : ord(u) <= t).
-// A NaN may pass the ord compare (its sel_key is 0 < T), so the prefilter
-// admits a superset; the exact per-element test is sel_key(u) >= T.
-template <int DT, bool LARGEST>
-RK_DEV int32_t ord_threshold(uint32_t T) {
-  constexpr int KB = Traits<DT>::KEY_BITS;
-  constexpr uint32_t KM = Traits<DT>::KEY_MASK, TOP = 1u << (KB - 1);
-  const uint32_t f = (LARGEST ? T : ~T) & KM;  // the flip at the boundary
-  const uint32_t o = f ^ TOP;
-  if constexpr (KB == 32) return (int32_t)o;
-  else return (int32_t)(int16_t)(uint16_t)o;
-}
-
-// Best element of a 16-byte vector in ord space (max for largest-k, min for
-// smallest-k): the vector can hold a key >= T only if this passes the
-// ord_threshold compare, one integer compare per vector.
-typedef short rk_s16x2 __attribute__((ext_vector_type(2)));
-template <int DT, bool LARGEST>
-RK_DEV int32_t vec_best_ord(const uint4& x) {
+typedef unsigned short rk_u16x2 __attribute__((ext_vector_type(2)));
+template <int DT>
+RK_DEV uint32_t vec_max_xm(const uint4& x, uint32_t m) {
   if constexpr (DT == F32) {
-    const int32_t a = ord32(x.x), b = ord32(x.y), c = ord32(x.z), d = ord32(x.w);
-    return LARGEST ? max(max(a, b), max(c, d)) : min(min(a, b), min(c, d));
+    return max(max(x.x ^ m, x.y ^ m), max(x.z ^ m, x.w ^ m));
   } else {
-    auto h = [](uint32_t w) { return __builtin_bit_cast(rk_s16x2, ord16x2(w)); };
-    const rk_s16x2 a = h(x.x), b = h(x.y), c = h(x.z), d = h(x.w);
-    rk_s16x2 m;
-    if constexpr (LARGEST)
-      m = __builtin_elementwise_max(__builtin_elementwise_max(a, b), __builtin_elementwise_max(c, d));
-    else
-      m = __builtin_elementwise_min(__builtin_elementwise_min(a, b), __builtin_elementwise_min(c, d));
-    const int32_t lo = m.x, hi = m.y;
-    return LARGEST ? max(lo, hi) : min(lo, hi);
+    auto h = [&](uint32_t w) { return __builtin_bit_cast(rk_u16x2, w ^ m); };
+    const rk_u16x2 r = __builtin_elementwise_max(__builtin_elementwise_max(h(x.x), h(x.y)),
+                                                 __builtin_elementwise_max(h(x.z), h(x.w)));
+    return max((uint32_t)r.x, (uint32_t)r.y);
   }
 }
 

@@ -528,21 +528,6 @@ __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
     if (ng < ng_min) ng = ng_min;                   // but at least 64 runs,
     if (ng > ng_cap) ng = ng_cap;                   // and never beyond the LDS
     if (ng < 1) ng = 1;
-    // A smaller sample where it suffices: the runs are halved while the
-    // candidates expected at the resulting threshold, len * r(s) / s, stay
-    // within max(2k, k + 2048) and within half the select's LDS keys (fewer
-    // keys to load and select per row; e.g. one 2^20 row with k = 256 samples
-    // 4096 keys, expecting ~1800 candidates).
-    {
-      const float kf = (float)d.k, lim = fminf(fmaxf(2.0f * kf, kf + 2048.0f), 0.5f * p.lds_keys);
-      while (ng >= 128) {
-        const float s2 = (float)((ng / 2) * ge);
-        const float mu2 = s2 * kf / (float)len;
-        const float r2 = ceilf(mu2 + 4.0f * sqrtf(mu2) + 2.0f);
-        if ((float)len * r2 / s2 >= lim) break;
-        ng /= 2;
-      }
-    }
     const uint32_t s = (uint32_t)(ng * ge);
     // run g starts at floor(g * (len - ge) / (ng - 1)) (in double: no 64-bit
     // integer division per key), clamped so that the run stays in the row
@@ -608,6 +593,14 @@ __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
 // a full stage is flushed by the wave alone with one global atomic on the
 // row's counter: no workgroup barrier anywhere in the streaming loop.
 constexpr int WSTAGE = 512;
+// Second level: a vector that the prefilter passes is staged whole (16 bytes
+// and the row position of its first element) in a per-wave LDS stage of
+// VSTAGE vectors; when it is full (and at the segment's end) each lane takes
+// one staged vector, computes its exact element mask and appends the keys.
+// The streaming loop then spends a few instructions per hit vector instead
+// of the per-element masks and ballots with most lanes idle (hits are
+// sparse: ~0.1-0.5 % of vectors on the batched workloads).
+constexpr int VSTAGE = 64;
 
 struct AppendCtx {
   uint64_t* stage;     // this wave's LDS stage (WSTAGE entries)
@@ -659,16 +652,19 @@ RK_DEV void append_vec(const AppendCtx& ac, uint32_t& n, uint4 x, uint32_t mask,
   n += tot;
 }
 
-// At most 72 VGPRs (7 waves per SIMD) for 2 and 4 loads in flight: the
-// rarely taken candidate path would otherwise cost the 16-bit variants
-// occupancy (83 VGPRs, 5 waves); the limit causes no spills there.
+// At most 72 VGPRs (7 waves per SIMD) for 2 loads in flight and 80 (6
+// waves) for 4, so that the rarely taken candidate path costs no occupancy
+// and no spills.
 template <int DT, bool LARGEST, int U>
-__global__ __launch_bounds__(STREAM_NT) __attribute__((amdgpu_waves_per_eu(U >= 8 ? 1 : 7)))
+__global__ __launch_bounds__(STREAM_NT)
+__attribute__((amdgpu_waves_per_eu(U >= 8 ? 1 : (U == 4 ? 6 : 7))))
 void k_filter(KParams p) {
   if (p.round && p.hdr->fallback_rows == 0) return;  // nothing fell back
   using UT = typename Traits<DT>::U;
   constexpr int P = Traits<DT>::PACK;
   __shared__ uint64_t stage[STREAM_NT / 64][WSTAGE];
+  __shared__ uint4 vstage[STREAM_NT / 64][VSTAGE];
+  __shared__ uint32_t vpos[STREAM_NT / 64][VSTAGE];
   __shared__ uint32_t flush_n[STREAM_NT / 64], flush_base[STREAM_NT / 64];
   const int lane = lane_id();
   // wave-uniform (readfirstlane tells the compiler), so the streaming loop's
@@ -684,7 +680,7 @@ void k_filter(KParams p) {
     const uint32_t T = st.T;
     const uint32_t cap = st.cap;
     const bool all = (T == 0);
-    const int32_t to = all ? 0 : ord_threshold<DT, LARGEST>(T);
+    const uint32_t xm = prefilter_mask<DT, LARGEST>(T);
     const UT* rb = static_cast<const UT*>(d.base);
     // 16-byte vectors are counted from a0, the row's first element on a
     // 128-byte boundary (an L2 line): a wave's 1 KB load then covers 8
@@ -713,6 +709,27 @@ void k_filter(KParams p) {
     ac.cand_cnt = &p.cand_cnt[r];
     ac.cap = cap;
     uint32_t staged = 0;  // keys in this wave's stage (wave-uniform)
+    uint4* vst = vstage[wave_base >> 6];
+    uint32_t* vps = vpos[wave_base >> 6];
+    uint32_t vstaged = 0;  // vectors in this wave's vector stage (wave-uniform)
+    // Exact masks of the staged vectors, one per lane, and their keys
+    // appended (called by the whole wave).
+    auto flush_vecs = [&]() {
+      if (vstaged == 0) return;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      uint4 xv = make_uint4(0u, 0u, 0u, 0u);
+      uint32_t first = 0, mm = 0;
+      if ((uint32_t)lane < vstaged) {
+        xv = vst[lane];
+        first = vps[lane];
+        const Vec<DT> v = unpack<DT>(xv);
+#pragma unroll
+        for (int c = 0; c < P; ++c) mm |= (sel_key<DT, LARGEST>(v.e[c]) >= T ? 1u : 0u) << c;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      if (__ballot(mm != 0)) append_vec<DT, LARGEST>(ac, staged, xv, mm, (int64_t)first);
+      vstaged = 0;
+    };
 
     // The row's head and tail elements outside the vector grid, one per
     // lane: wave 0 of the first segment and the last wave of the last.
@@ -756,23 +773,26 @@ void k_filter(KParams p) {
           for (int u = 0; u < U; ++u) x[u] = bload16_nt(rs, (jb + lane + u * STREAM_NT) * 16u);
         };
         auto test_set = [&](const uint4 (&x)[U], uint32_t jb) {
-          uint32_t hb = 0;  // bit u: this lane's vector u holds a candidate
+          uint32_t hb = 0;  // bit u: this lane's vector u may hold a candidate
 #pragma unroll
           for (int u = 0; u < U; ++u) {
-            const int32_t b = vec_best_ord<DT, LARGEST>(x[u]);
-            hb |= ((LARGEST ? (b >= to) : (b <= to)) ? 1u : 0u) << u;
+            const uint32_t j = jb + lane + u * STREAM_NT;
+            hb |= ((vec_max_xm<DT>(x[u], xm) >= T && j < nvc) ? 1u : 0u) << u;
           }
-          if (__ballot(hb != 0)) {  // rare: exact per-element masks, only for
-#pragma unroll                      // the vector slots some lane hit
-            for (int u = 0; u < U; ++u) {
-              if (!__ballot((hb >> u) & 1u)) continue;
-              const uint32_t j = jb + lane + u * STREAM_NT;
-              const Vec<DT> v = unpack<DT>(x[u]);
-              uint32_t mm = 0;
+          if (__ballot(hb != 0)) {  // rare: stage the vectors that passed
 #pragma unroll
-              for (int c = 0; c < P; ++c) mm |= (sel_key<DT, LARGEST>(v.e[c]) >= T ? 1u : 0u) << c;
-              mm = (j < nvc && ((hb >> u) & 1u)) ? mm : 0u;
-              if (__ballot(mm != 0)) append_vec<DT, LARGEST>(ac, staged, x[u], mm, al + (c0 + j) * P);
+            for (int u = 0; u < U; ++u) {
+              const bool hit = (hb >> u) & 1u;
+              const uint64_t bm = __ballot(hit);
+              if (!bm) continue;
+              const uint32_t nb = (uint32_t)__popcll(bm);
+              if (vstaged + nb > (uint32_t)VSTAGE) flush_vecs();
+              if (hit) {
+                const uint32_t sl = vstaged + mbcnt(bm);
+                vst[sl] = x[u];
+                vps[sl] = (uint32_t)(al + (int64_t)(c0 + jb + lane + u * STREAM_NT) * P);
+              }
+              vstaged += nb;
             }
           }
         };
@@ -793,6 +813,7 @@ void k_filter(KParams p) {
         }
       }
     }
+    flush_vecs();
     // End of the segment: one atomic per workgroup reserves the slots of
     // all four waves' staged keys (per-wave atomics on the row's counter,
     // from every workgroup reading that row, serialized at the counter).
