@@ -85,6 +85,18 @@ int rk_topk_batched(int dtype, const void* vals, const int64_t* row_off,
                     void* out_vals, int32_t* out_idx, const int64_t* out_off,
                     void* workspace, size_t ws_bytes, void* stream);
 
+/* rk_topk_batched with 64-bit indices: idx_in (optional) and out_idx are
+ * int64_t, the reference's IdxType = int64_t instantiation of
+ * topKRadixSelectL<IdxType, ...>(valIn, idxIn, valOut, idxOut, ...)
+ * (topk_radixselect.h:59-70).  Row lengths stay below 2^31 (positions
+ * written without idx_in are row-local); the tail fill index is -1.  Same
+ * workspace as rk_topk_batched. */
+int rk_topk_batched_i64(int dtype, const void* vals, const int64_t* row_off,
+                        const int32_t* row_k, int32_t num_rows, int64_t total_len,
+                        int32_t max_k, uint32_t flags, const int64_t* idx_in,
+                        void* out_vals, int64_t* out_idx, const int64_t* out_off,
+                        void* workspace, size_t ws_bytes, void* stream);
+
 /* Allocation-free form of rk_topk_batched.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory or hand in
  * a workspace (code under hipGraph capture, tight serving loops, harnesses
@@ -131,6 +143,13 @@ int rk_topk_compat(int dtype, const void* valIn, const int32_t* idxIn,
                    void* valOut, int32_t* idxOut, void* workspace,
                    size_t ws_bytes, const int32_t* taskLen, int32_t taskNum,
                    int32_t K, uint32_t flags, void* stream);
+
+/* rk_topk_compat with IdxType = int64_t: idxIn (optional) and idxOut are
+ * int64_t (topk_radixselect.h:59-70 instantiated with int64_t indices). */
+int rk_topk_compat_i64(int dtype, const void* valIn, const int64_t* idxIn,
+                       void* valOut, int64_t* idxOut, void* workspace,
+                       size_t ws_bytes, const int32_t* taskLen, int32_t taskNum,
+                       int32_t K, uint32_t flags, void* stream);
 
 /* Waits for `stream`, then reads the device status of the last call that
  * used `workspace`: 0 when every row was valid.  Optional outputs: rows that

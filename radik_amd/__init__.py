@@ -55,8 +55,8 @@ _ERRORS = {
 
 # Every symbol include/radik_amd.h and include/radik_amd_harness.h declare.
 EXPORTED = (
-    "rk_max_k", "rk_topk_workspace_size", "rk_topk_batched", "rk_topk_batched_noalloc",
-    "rk_topk_rows",
+    "rk_max_k", "rk_topk_workspace_size", "rk_topk_batched", "rk_topk_batched_i64",
+    "rk_topk_batched_noalloc", "rk_topk_rows", "rk_topk_compat_i64",
     "rk_compat_workspace_size", "rk_topk_compat", "rk_topk_status", "rk_strerror",
     "rk_get_tune", "rk_set_tune", "rk_gen_fill", "rk_timing_enable", "rk_timing_read",
     "rk_autotune_batched", "rk_tune_cache_save", "rk_tune_cache_load", "rk_tune_cache_clear",
@@ -126,6 +126,9 @@ def lib():
     L.rk_topk_workspace_size.argtypes = [c.c_int, i32, i64, i32, u32, c.POINTER(sz)]
     L.rk_topk_batched.argtypes = [c.c_int, vp, vp, vp, i32, i64, i32, u32, vp, vp, vp, vp,
                                   vp, sz, vp]
+    L.rk_topk_batched_i64.argtypes = [c.c_int, vp, vp, vp, i32, i64, i32, u32, vp, vp, vp, vp,
+                                      vp, sz, vp]
+    L.rk_topk_compat_i64.argtypes = [c.c_int, vp, vp, vp, vp, vp, sz, vp, i32, i32, u32, vp]
     L.rk_topk_batched_noalloc.argtypes = [c.c_int, vp, vp, vp, i32, i64, i32, u32, vp, vp, vp,
                                           vp, vp]
     L.rk_topk_rows.argtypes = [c.c_int, vp, vp, vp, i32, i64, i32, u32, vp, vp, vp, sz, vp]
@@ -255,45 +258,56 @@ def _alloc_ws(nbytes: int, device):
 
 def topk_radix_select(valIn, K: int, taskLen, *, idxIn=None, largest: bool = False,
                       ascend: bool = True, need_sorting: bool = True, exact: bool = False,
-                      workspace=None, valOut=None, idxOut=None, stream=None):
+                      workspace=None, valOut=None, idxOut=None, stream=None,
+                      idx_dtype=None):
     """Mirror of topKRadixSelectL<IdxType, LARGEST, ASCEND, ...>(valIn, idxIn,
     valOut, idxOut, workSpace, taskLen, taskNum, K, stream, needSorting).
 
     The reference's defaults in its harness are smallest-k, ascending
     (radik/test/test_radik.cu:18-20); they are kept here.  Returns
     ``(valOut, idxOut)`` shaped ``[taskNum, K]``.  WITHSCALE has no switch:
-    the key-space scaling is always exact and always on.
+    the key-space scaling is always exact and always on.  IdxType is int32,
+    or int64 (rk_topk_compat_i64) when ``idx_dtype`` is torch.int64 or an
+    int64 ``idxIn`` / ``idxOut`` is given.
     """
     torch = _torch()
     taskLen = [int(x) for x in taskLen]
     taskNum = len(taskLen)
     dt = _dtype_code(valIn)
+    if idx_dtype is None:
+        idx_dtype = torch.int64 if any(t is not None and t.dtype == torch.int64
+                                       for t in (idxIn, idxOut)) else torch.int32
+    i64 = idx_dtype == torch.int64
     if valOut is None:
         valOut = torch.empty((taskNum, K), dtype=valIn.dtype, device=valIn.device)
     if idxOut is None:
-        idxOut = torch.empty((taskNum, K), dtype=torch.int32, device=valIn.device)
+        idxOut = torch.empty((taskNum, K), dtype=idx_dtype, device=valIn.device)
     if workspace is None:
         workspace = _alloc_ws(get_workspace_size(K, max(taskLen, default=0), taskNum, dt),
                               valIn.device)
     tl = (ctypes.c_int32 * max(taskNum, 1))(*taskLen)
     flags = _flags(largest, need_sorting, ascend, exact)
-    rc = lib().rk_topk_compat(dt, _ptr(valIn), _ptr(idxIn), _ptr(valOut), _ptr(idxOut),
-                              _ptr(workspace), workspace.numel(), tl, taskNum, K, flags,
-                              _stream_ptr(stream))
-    _check(rc, "rk_topk_compat")
+    fn = lib().rk_topk_compat_i64 if i64 else lib().rk_topk_compat
+    rc = fn(dt, _ptr(valIn), _ptr(idxIn), _ptr(valOut), _ptr(idxOut), _ptr(workspace),
+            workspace.numel(), tl, taskNum, K, flags, _stream_ptr(stream))
+    _check(rc, "rk_topk_compat_i64" if i64 else "rk_topk_compat")
     return valOut, idxOut
 
 
 def topk_batched(values, row_off, row_k, *, largest: bool = True, sorted: bool = True,
                  ascend: bool | None = None, exact: bool = False, idx_in=None, out_off=None,
                  total_len: int | None = None, max_k_: int | None = None, workspace=None,
-                 out_vals=None, out_idx=None, stream=None):
+                 out_vals=None, out_idx=None, stream=None, idx64: bool | None = None):
     """Uneven batched top-k.  ``values`` 1-D device tensor of concatenated rows,
     ``row_off`` int64 [R+1] and ``row_k`` int32 [R] on the device.  Output row
     r occupies ``[out_off[r], out_off[r+1])`` (default: packed by min(len, k)).
     Host bounds ``total_len`` / ``max_k_`` default to the exact values (which
-    reads the metadata back; pass them to stay asynchronous)."""
+    reads the metadata back; pass them to stay asynchronous).  Indices are
+    int32, or int64 (rk_topk_batched_i64) when ``idx64`` is set or an int64
+    ``idx_in`` / ``out_idx`` is given."""
     torch = _torch()
+    if idx64 is None:
+        idx64 = any(t is not None and t.dtype == torch.int64 for t in (idx_in, out_idx))
     dt = _dtype_code(values)
     R = row_k.numel()
     if total_len is None:
@@ -308,16 +322,19 @@ def topk_batched(values, row_off, row_k, *, largest: bool = True, sorted: bool =
     n_out = int(out_off[-1].item()) if out_vals is None or out_idx is None else 0
     if out_vals is None:
         out_vals = torch.empty(max(n_out, 1), dtype=values.dtype, device=values.device)
+    idt = torch.int64 if idx64 else torch.int32
     if out_idx is None:
-        out_idx = torch.empty(max(n_out, 1), dtype=torch.int32, device=values.device)
+        out_idx = torch.empty(max(n_out, 1), dtype=idt, device=values.device)
+    if out_idx.dtype != idt or (idx_in is not None and idx_in.dtype != idt):
+        raise TypeError(f"topk_batched: index tensors must be {idt}")
     flags = _flags(largest, sorted, ascend, exact)
     if workspace is None:
         workspace = _alloc_ws(workspace_size(dt, R, total_len, max_k_, flags), values.device)
-    rc = lib().rk_topk_batched(dt, _ptr(values), _ptr(row_off), _ptr(row_k), R, total_len,
-                               max_k_, flags, _ptr(idx_in), _ptr(out_vals), _ptr(out_idx),
-                               _ptr(out_off), _ptr(workspace), workspace.numel(),
-                               _stream_ptr(stream))
-    _check(rc, "rk_topk_batched")
+    fn = lib().rk_topk_batched_i64 if idx64 else lib().rk_topk_batched
+    rc = fn(dt, _ptr(values), _ptr(row_off), _ptr(row_k), R, total_len, max_k_, flags,
+            _ptr(idx_in), _ptr(out_vals), _ptr(out_idx), _ptr(out_off), _ptr(workspace),
+            workspace.numel(), _stream_ptr(stream))
+    _check(rc, "rk_topk_batched_i64" if idx64 else "rk_topk_batched")
     return out_vals, out_idx, out_off, workspace
 
 

@@ -130,7 +130,7 @@ Tune current_tune(const DevInfo& di, int dtype, uint32_t flags, int32_t rows, in
 // Shared body of the batched entry points.
 struct Outputs {
   void* vals = nullptr;
-  int32_t* idx = nullptr;
+  void* idx = nullptr;  // int32, or int64 when idx64
   const int64_t* off = nullptr;
   void* const* val_ptrs = nullptr;
   int32_t* const* idx_ptrs = nullptr;
@@ -138,9 +138,9 @@ struct Outputs {
 
 int launch(int dtype, const void* vals, const int64_t* row_off, const void* const* row_ptrs,
            const int64_t* row_len, const int32_t* row_k, int32_t K, int32_t num_rows,
-           int64_t total_len, int32_t max_k, uint32_t flags, const int32_t* idx_in,
+           int64_t total_len, int32_t max_k, uint32_t flags, const void* idx_in,
            const Outputs& out, void* workspace, size_t ws_bytes, void* stream,
-           int part = rk::PART_ALL, uint32_t* publish = nullptr) {
+           int part = rk::PART_ALL, uint32_t* publish = nullptr, bool idx64 = false) {
   if (!valid_dtype(dtype)) return RK_E_UNSUPPORTED;
   if (num_rows < 0 || total_len < 0 || max_k < 0) return RK_E_ARG;
   if (num_rows == 0) return RK_OK;
@@ -153,6 +153,8 @@ int launch(int dtype, const void* vals, const int64_t* row_off, const void* cons
   if (max_k > di.lds_keys && (flags & RK_SORTED)) return RK_E_UNSUPPORTED;
   const int es = elem_size(dtype);
   if (((uintptr_t)vals % es) || ((uintptr_t)out_vals % es)) return RK_E_ALIGN;
+  const size_t isz = idx64 ? 8 : 4;
+  if (((uintptr_t)idx_in % isz) || ((uintptr_t)out.idx % isz)) return RK_E_ALIGN;
   if (!workspace) return RK_E_WORKSPACE;
   if ((uintptr_t)workspace % 256) return RK_E_ALIGN;
   const Layout L = make_layout(num_rows, total_len, max_k);
@@ -171,6 +173,7 @@ int launch(int dtype, const void* vals, const int64_t* row_off, const void* cons
   p.out_idx_ptrs = out.idx_ptrs;
   p.out_vals = out.vals;
   p.out_idx = out.idx;
+  p.idx64 = idx64 ? 1 : 0;
   p.K = K;
   p.R = num_rows;
   p.max_k = max_k;
@@ -312,6 +315,21 @@ int rk_topk_batched(int dtype, const void* vals, const int64_t* row_off, const i
                 flags, idx_in, o, workspace, ws_bytes, stream);
 }
 
+int rk_topk_batched_i64(int dtype, const void* vals, const int64_t* row_off,
+                        const int32_t* row_k, int32_t num_rows, int64_t total_len,
+                        int32_t max_k, uint32_t flags, const int64_t* idx_in, void* out_vals,
+                        int64_t* out_idx, const int64_t* out_off, void* workspace,
+                        size_t ws_bytes, void* stream) {
+  if (!valid_dtype(dtype)) return RK_E_UNSUPPORTED;
+  if (!row_k || !out_off) return RK_E_ARG;
+  Outputs o;
+  o.vals = out_vals;
+  o.idx = out_idx;
+  o.off = out_off;
+  return launch(dtype, vals, row_off, nullptr, nullptr, row_k, 0, num_rows, total_len, max_k,
+                flags, idx_in, o, workspace, ws_bytes, stream, rk::PART_ALL, nullptr, true);
+}
+
 int rk_topk_rows(int dtype, const void* const* row_ptrs, const int64_t* row_len,
                  const int32_t* row_k, int32_t num_rows, int64_t total_len, int32_t max_k,
                  uint32_t flags, void* const* out_val_ptrs, int32_t* const* out_idx_ptrs,
@@ -336,9 +354,13 @@ int rk_compat_workspace_size(int dtype, int32_t K, int32_t n, int32_t taskNum, u
   return RK_OK;
 }
 
-int rk_topk_compat(int dtype, const void* valIn, const int32_t* idxIn, void* valOut,
-                   int32_t* idxOut, void* workspace, size_t ws_bytes, const int32_t* taskLen,
-                   int32_t taskNum, int32_t K, uint32_t flags, void* stream) {
+}  // extern "C"
+
+namespace {
+
+int topk_compat(int dtype, const void* valIn, const void* idxIn, void* valOut, void* idxOut,
+                void* workspace, size_t ws_bytes, const int32_t* taskLen, int32_t taskNum,
+                int32_t K, uint32_t flags, void* stream, bool idx64) {
   if (!taskLen || taskNum < 0 || K < 0) return RK_E_ARG;
   if (!workspace) return RK_E_WORKSPACE;  // topk_radixselect.h:77-79
   if (taskNum == 0) return RK_OK;
@@ -365,7 +387,25 @@ int rk_topk_compat(int dtype, const void* valIn, const int32_t* idxIn, void* val
   o.idx = idxOut;
   return launch(dtype, valIn, d_off, nullptr, nullptr, nullptr, K, taskNum,
                 (int64_t)longest * taskNum, K, flags, idxIn, o, ws + off_bytes,
-                ws_bytes - off_bytes, stream);
+                ws_bytes - off_bytes, stream, rk::PART_ALL, nullptr, idx64);
+}
+
+}  // namespace
+
+extern "C" {
+
+int rk_topk_compat(int dtype, const void* valIn, const int32_t* idxIn, void* valOut,
+                   int32_t* idxOut, void* workspace, size_t ws_bytes, const int32_t* taskLen,
+                   int32_t taskNum, int32_t K, uint32_t flags, void* stream) {
+  return topk_compat(dtype, valIn, idxIn, valOut, idxOut, workspace, ws_bytes, taskLen, taskNum,
+                     K, flags, stream, false);
+}
+
+int rk_topk_compat_i64(int dtype, const void* valIn, const int64_t* idxIn, void* valOut,
+                       int64_t* idxOut, void* workspace, size_t ws_bytes, const int32_t* taskLen,
+                       int32_t taskNum, int32_t K, uint32_t flags, void* stream) {
+  return topk_compat(dtype, valIn, idxIn, valOut, idxOut, workspace, ws_bytes, taskLen, taskNum,
+                     K, flags, stream, true);
 }
 
 int rk_topk_batched_noalloc(int dtype, const void* vals, const int64_t* row_off,

@@ -224,6 +224,23 @@ RK_DEV int64_t aligned_start(const typename Traits<DT>::U* rb, int64_t es) {
   return es + (int64_t)(((16u - (addr & 15u)) & 15u) / ES);
 }
 
+// Output indices are int32 or int64 (the reference's IdxType,
+// topk_radixselect.h:59-63), chosen per call (KParams.idx64): the caller's
+// index of row position pos, and the stores into the row's index slots.
+RK_DEV int64_t caller_idx(const KParams& p, const RowDesc& d, uint32_t pos) {
+  if (!d.idx_in) return (int64_t)pos;
+  return p.idx64 ? ldg(static_cast<const int64_t*>(d.idx_in) + pos)
+                 : (int64_t)ldg(static_cast<const int32_t*>(d.idx_in) + pos);
+}
+RK_DEV void put_idx(const KParams& p, const RowDesc& d, int64_t o, int64_t v) {
+  if (p.idx64) static_cast<int64_t*>(d.out_i)[o] = v;
+  else static_cast<int32_t*>(d.out_i)[o] = (int32_t)v;
+}
+RK_DEV int64_t get_idx(const KParams& p, const RowDesc& d, int64_t o) {
+  return p.idx64 ? static_cast<const int64_t*>(d.out_i)[o]
+                 : (int64_t) static_cast<const int32_t*>(d.out_i)[o];
+}
+
 // ---------------------------------------------------------------------------
 // plan
 // ---------------------------------------------------------------------------
@@ -240,7 +257,8 @@ template <int DT>
 RK_DEV RowPlan plan_row(const KParams& p, int r) {
   using U = typename Traits<DT>::U;
   const U* base;
-  const int32_t* idx = nullptr;
+  const void* idx = nullptr;
+  const int64_t isz = p.idx64 ? 8 : 4;  // bytes per index
   int64_t len;
   if (p.row_ptrs) {
     base = static_cast<const U*>(p.row_ptrs[r]);
@@ -249,7 +267,7 @@ RK_DEV RowPlan plan_row(const KParams& p, int r) {
     const int64_t off = p.row_off[r];
     len = p.row_off[r + 1] - off;
     base = static_cast<const U*>(p.vals) + off;
-    if (p.idx_in) idx = p.idx_in + off;
+    if (p.idx_in) idx = static_cast<const char*>(p.idx_in) + off * isz;
   }
   const int64_t k = p.row_k ? p.row_k[r] : p.K;
   int64_t slots;
@@ -262,7 +280,7 @@ RK_DEV RowPlan plan_row(const KParams& p, int r) {
     const int64_t out_pos = p.out_off ? p.out_off[r] : (int64_t)r * p.K;
     slots = p.out_off ? p.out_off[r + 1] - p.out_off[r] : p.K;
     d.out_v = static_cast<U*>(p.out_vals) + out_pos;
-    d.out_i = p.out_idx + out_pos;
+    d.out_i = static_cast<char*>(p.out_idx) + out_pos * isz;
   }
   const int64_t m = len < k ? len : k;
   d.base = base;
@@ -900,17 +918,16 @@ RK_DEV void sort_and_write(const KParams& p, const RowDesc& d, uint64_t* sbuf, u
   (void)r;
   const U* rb = static_cast<const U*>(d.base);
   U* ov = static_cast<U*>(d.out_v);
-  int32_t* oi = d.out_i;
   for (uint32_t i = threadIdx.x; i < m; i += SEL_NT) {
     uint32_t sk, pos;
     order_decode<DT>(mode, sbuf[i], sk, pos);
     const uint32_t bits = sk ? key_to_bits<DT, LARGEST>(sk) : (uint32_t)ldg(rb + pos);
     ov[i] = (U)bits;
-    oi[i] = d.idx_in ? d.idx_in[pos] : (int32_t)pos;
+    put_idx(p, d, i, caller_idx(p, d, pos));
   }
   for (int64_t i = (int64_t)m + threadIdx.x; i < d.slots; i += SEL_NT) {
     ov[i] = (U)0;
-    oi[i] = -1;
+    put_idx(p, d, i, -1);
   }
 }
 
@@ -957,11 +974,11 @@ void k_select_finalize(KParams p) {
       U* ov = static_cast<U*>(d.out_v);
       for (uint32_t i = threadIdx.x; i < m; i += SEL_NT) {
         ov[i] = ldg(rb + i);
-        d.out_i[i] = d.idx_in ? d.idx_in[i] : (int32_t)i;
+        put_idx(p, d, i, caller_idx(p, d, i));
       }
       for (int64_t i = (int64_t)m + threadIdx.x; i < d.slots; i += SEL_NT) {
         ov[i] = (U)0;
-        d.out_i[i] = -1;
+        put_idx(p, d, i, -1);
       }
     } else if (ph == PH_COPYALL) {
       const uint32_t m = (uint32_t)(len < (int64_t)k ? len : (int64_t)k);
@@ -1244,12 +1261,12 @@ __global__ __launch_bounds__(SEL_NT) void k_merge_rank(KParams p) {
       order_decode<DT>(mode, x[j], sk, pos);
       const uint32_t bits = sk ? key_to_bits<DT, LARGEST>(sk) : (uint32_t)ldg(rb + pos);
       ov[rank[j]] = (U)bits;
-      d.out_i[rank[j]] = d.idx_in ? d.idx_in[pos] : (int32_t)pos;
+      put_idx(p, d, rank[j], caller_idx(p, d, pos));
     }
     if (c == 0) {
       for (int64_t i = (int64_t)k + threadIdx.x; i < d.slots; i += SEL_NT) {
         ov[i] = (U)0;
-        d.out_i[i] = -1;
+        put_idx(p, d, i, -1);
       }
     }
     __syncthreads();
@@ -1515,7 +1532,6 @@ __global__ __launch_bounds__(STREAM_NT) void k_emit_write(KParams p) {
     const uint32_t kk = (uint32_t)d.k;
     const UT* rb = static_cast<const UT*>(d.base);
     UT* ov = static_cast<UT*>(d.out_v);
-    int32_t* oi = d.out_i;
     // Keys > V and == V in the row's tiles before this segment (the counts
     // k_emit_count wrote); within the segment the running totals carry on.
     const uint2* tc = p.tile_cnt + p.tile_start[r];
@@ -1561,7 +1577,7 @@ __global__ __launch_bounds__(STREAM_NT) void k_emit_write(KParams p) {
             const uint32_t o = gb + (qb < rem ? qb : rem);
             if (o < kk) {
               ov[o] = (UT)v.e[c];
-              oi[o] = (int32_t)pos;
+              put_idx(p, d, o, pos);  // a position: mapped by k_finalize_ordered
             }
             ++gb;
           } else if (sk[c] == V) {
@@ -1569,7 +1585,7 @@ __global__ __launch_bounds__(STREAM_NT) void k_emit_write(KParams p) {
               const uint32_t o = gb + qb;
               if (o < kk) {
                 ov[o] = (UT)v.e[c];
-                oi[o] = (int32_t)pos;
+                put_idx(p, d, o, pos);
               }
             }
             ++qb;
@@ -1596,18 +1612,18 @@ __global__ __launch_bounds__(SEL_NT) void k_finalize_ordered(KParams p) {
       // the emit wrote values and positions in position order already: map
       // positions through idx_in and write the tail fill (any k, no LDS)
       U* ow = static_cast<U*>(d.out_v);
-      int32_t* iw = d.out_i;
       if (d.idx_in)
-        for (uint32_t i = threadIdx.x; i < m; i += SEL_NT) iw[i] = d.idx_in[iw[i]];
+        for (uint32_t i = threadIdx.x; i < m; i += SEL_NT)
+          put_idx(p, d, i, caller_idx(p, d, (uint32_t)get_idx(p, d, i)));
       for (int64_t i = (int64_t)m + threadIdx.x; i < d.slots; i += SEL_NT) {
         ow[i] = (U)0;
-        iw[i] = -1;
+        put_idx(p, d, i, -1);
       }
     } else {
       const U* ov = static_cast<const U*>(d.out_v);
-      const int32_t* oi = d.out_i;
       for (uint32_t i = threadIdx.x; i < m; i += SEL_NT)
-        sbuf[i] = order_key<DT>(mode, sel_key<DT, LARGEST>((uint32_t)ov[i]), (uint32_t)oi[i]);
+        sbuf[i] = order_key<DT>(mode, sel_key<DT, LARGEST>((uint32_t)ov[i]),
+                                (uint32_t)get_idx(p, d, i));
       __syncthreads();
       sort_and_write<DT, LARGEST>(p, d, sbuf, m, mode);
     }

@@ -38,10 +38,10 @@ enum Status : uint32_t {
 };
 
 struct RowDesc {
-  const void* base;       // first element of the row
-  const int32_t* idx_in;  // optional caller indices, parallel to base
-  void* out_v;            // first output value slot of the row
-  int32_t* out_i;         // first output index slot of the row
+  const void* base;    // first element of the row
+  const void* idx_in;  // optional caller indices, parallel to base (int32 or int64)
+  void* out_v;         // first output value slot of the row
+  void* out_i;         // first output index slot of the row (int32 or int64)
   int64_t len;
   int32_t k;
   int32_t slots;  // output slots reserved for the row (>= min(len, k))
@@ -97,13 +97,14 @@ struct KParams {
   const int64_t* row_off;
   const int32_t* row_k;
   const int64_t* out_off;
-  const int32_t* idx_in;
+  const void* idx_in;           // int32, or int64 when idx64
   const void* const* row_ptrs;  // pointer form (serving)
   const int64_t* row_len;       // pointer form (serving)
   void* const* out_val_ptrs;    // pointer form: per-row output values
   int32_t* const* out_idx_ptrs; // pointer form: per-row output indices
   void* out_vals;
-  int32_t* out_idx;
+  void* out_idx;  // int32, or int64 when idx64
+  int32_t idx64;  // caller indices and output indices are int64 (IdxType = int64_t)
   int32_t K;      // k for every row when row_k is null; slot stride when out_off is null
   int32_t R;      // rows
   int32_t max_k;

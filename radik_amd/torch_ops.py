@@ -47,11 +47,13 @@ def _run(values, offsets, ks, num_rows, total, max_k, flags, out_v, out_i, out_o
     ws = torch.empty(workspace_size(dt, num_rows, total, max_k, flags), dtype=torch.uint8,
                      device=values.device)
     cur = torch.cuda.current_stream(values.device)
-    rc = lib().rk_topk_batched(dt, values.data_ptr(), offsets.data_ptr(), ks.data_ptr(),
-                               num_rows, total, max_k, flags, None, out_v.data_ptr(),
-                               out_i.data_ptr(), out_off.data_ptr(), ws.data_ptr(), ws.numel(),
-                               cur.cuda_stream)
-    _check(rc, "rk_topk_batched")
+    # int64 index outputs are written directly (rk_topk_batched_i64)
+    i64 = out_i.dtype == torch.int64
+    fn = lib().rk_topk_batched_i64 if i64 else lib().rk_topk_batched
+    rc = fn(dt, values.data_ptr(), offsets.data_ptr(), ks.data_ptr(), num_rows, total, max_k,
+            flags, None, out_v.data_ptr(), out_i.data_ptr(), out_off.data_ptr(), ws.data_ptr(),
+            ws.numel(), cur.cuda_stream)
+    _check(rc, "rk_topk_batched_i64" if i64 else "rk_topk_batched")
     if checked:
         st = _status(ws, cur)
         if st[0] != 0:
@@ -81,15 +83,14 @@ def topk(values: torch.Tensor, k: int, dim: int = -1, largest: bool = True,
     if rows >= 2 ** 31:
         raise ValueError("radik_amd::topk: more than 2^31 - 1 slices")
     out_v = torch.empty((*lead, k), dtype=values.dtype, device=values.device)
-    out_i32 = torch.empty((*lead, k), dtype=torch.int32, device=values.device)
+    out_i = torch.empty((*lead, k), dtype=torch.int64, device=values.device)
     if rows and k:
         dev = values.device
         off = torch.arange(0, rows + 1, dtype=torch.int64, device=dev) * n
         oo = torch.arange(0, rows + 1, dtype=torch.int64, device=dev) * k
         kk = torch.full((rows,), k, dtype=torch.int32, device=dev)
-        _run(x, off, kk, rows, rows * n, k, _flags(largest, sorted), out_v, out_i32, oo,
+        _run(x, off, kk, rows, rows * n, k, _flags(largest, sorted), out_v, out_i, oo,
              checked, "topk")
-    out_i = out_i32.to(torch.int64)
     if values.dim():
         out_v, out_i = out_v.movedim(-1, d), out_i.movedim(-1, d)
     else:
