@@ -251,6 +251,7 @@ typedef struct rk_server_stats {
   double issue_us;                        /* mean host time to issue one batch part */
   int32_t max_batch_rows;
   int32_t pad_;
+  int64_t retained; /* requests held: queued, in flight, or done and not yet waited for */
 } rk_server_stats;
 
 int rk_server_create(const rk_server_config* cfg, rk_server** out);
@@ -259,7 +260,11 @@ int rk_server_create(const rk_server_config* cfg, rk_server** out);
  * out_idx.  The row and outputs must stay valid until the request completes. */
 int rk_server_submit(rk_server* s, const void* vals, int64_t len, int32_t k, void* out_vals,
                      int32_t* out_idx, uint64_t* ticket);
-/* Blocks until the request completes; returns its status and latency. */
+/* Blocks until the request completes; returns its status and latency.  A
+ * request's outcome is reported once: the server then releases it (memory
+ * stays bounded by what is queued, in flight or not yet waited for), and a
+ * second wait on the ticket returns RK_E_ARG.  rk_server_replay reports and
+ * releases its own requests. */
 int rk_server_wait(rk_server* s, uint64_t ticket, double* latency_us);
 /* Blocks until nothing is queued or in flight. */
 int rk_server_drain(rk_server* s);
@@ -292,8 +297,12 @@ int rk_server_destroy(rk_server* s);
  *   flags: RK_SORTED (best first, ties by index) or 0 (database order).
  *   out_vals [nq][k] fp32 scores / distances, out_idx [nq][k] int32 indices.
  * rk_topk_status(workspace) afterwards reports RK_E_DEVICE if a query's
- * candidates at the safe threshold outgrew its slots (only with very many
- * scores tied at it); its fallback_rows output counts the queries that
+ * candidates at the safe threshold outgrew its slots: the safe threshold is
+ * the k-th best score of S evenly spaced database vectors (every floor(n/S)-th),
+ * so this happens when very many scores tie at it, or when the database's
+ * layout repeats with that stride so that the sampled vectors score far
+ * below the query's true k-th best.  Such a query's outputs are filled with
+ * value NaN and index -1.  The fallback_rows output counts the queries that
  * needed the second, safe-threshold pass. */
 enum { RK_KNN_IP = 0, RK_KNN_L2 = 1 };
 int rk_knn_workspace_size(int dtype, int32_t nq, int64_t n, int32_t d, int32_t k, size_t* bytes);

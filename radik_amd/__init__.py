@@ -95,7 +95,8 @@ class _ServerStats(ctypes.Structure):
                   "server_bytes", "peak_inflight_bytes")]
                 + [(n, ctypes.c_double) for n in
                    ("mean_batch_rows", "p50_us", "p99_us", "mean_us", "max_us", "issue_us")]
-                + [("max_batch_rows", ctypes.c_int32), ("pad_", ctypes.c_int32)])
+                + [("max_batch_rows", ctypes.c_int32), ("pad_", ctypes.c_int32),
+                   ("retained", ctypes.c_int64)])
 
 
 def lib_path() -> str:

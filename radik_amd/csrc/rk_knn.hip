@@ -540,6 +540,13 @@ __global__ __launch_bounds__(KNN_SEL_NT) void k_knn_select(int pass, const uint3
           atomicOr(&hdr->status, (uint32_t)ST_CANDIDATES);
         }
       }
+      if (pass != 0) {
+        // no exact result: the row reads as failed (value NaN, index -1)
+        for (uint32_t i = threadIdx.x; i < (uint32_t)k; i += KNN_SEL_NT) {
+          out_v[(size_t)q * k + i] = __builtin_nanf("");
+          out_i[(size_t)q * k + i] = -1;
+        }
+      }
       __syncthreads();
       continue;
     }

@@ -70,6 +70,7 @@ struct Request {
   double t_done = 0.0;
   uint64_t bytes = 0;
   int state = RQ_PENDING;
+  bool reported = false;  // rk_server_wait / rk_server_replay returned its outcome
 };
 
 // Per-row metadata of one batch: the five device arrays the rows entry point
@@ -138,8 +139,13 @@ struct rk_server {
 
   std::mutex mu;  // guards everything below
   std::condition_variable cv_work, cv_done;
-  std::deque<Request> reqs;      // indexed by ticket; references stay valid
-  std::deque<uint64_t> pending;  // tickets not yet admitted, arrival order
+  // Requests still to be reported, ticket t at reqs[t - base]; a request is
+  // retired from the front once its outcome has been reported, so memory
+  // stays bounded by what is queued, in flight or not yet waited for.
+  std::deque<Request> reqs;  // references stay valid while the deque grows
+  uint64_t base = 0;
+  std::vector<double> latencies_us;  // every completed request, for the statistics
+  std::deque<uint64_t> pending;      // tickets not yet admitted, arrival order
   uint64_t inflight = 0;         // request bytes admitted and not completed
   uint64_t peak_inflight = 0;
   int busy_slots = 0;
@@ -151,6 +157,17 @@ struct rk_server {
   int32_t max_rows_seen = 0;
 
   std::thread dispatcher;
+
+  Request& req(uint64_t ticket) { return reqs[ticket - base]; }
+  // Marks a request's outcome as reported and retires reported requests
+  // from the front.  Lock held.
+  void reported(uint64_t ticket) {
+    req(ticket).reported = true;
+    while (!reqs.empty() && reqs.front().reported && reqs.front().state >= RQ_DONE) {
+      reqs.pop_front();
+      ++base;
+    }
+  }
 
   void loop();
   bool start_batch(Slot& s);
@@ -195,7 +212,7 @@ bool rk_server::start_batch(Slot& s) {
     if (pending.empty() || error != RK_OK) return false;
     const uint64_t budget = (uint64_t)cfg.mem_budget - server_bytes;
     while (!pending.empty() && (int32_t)take.size() < cfg.max_batch_rows) {
-      Request& r = reqs[pending.front()];
+      Request& r = req(pending.front());
       if (r.bytes > budget) {  // can never be admitted
         r.state = RQ_REJECTED;
         r.t_done = now_s();
@@ -224,7 +241,7 @@ bool rk_server::start_batch(Slot& s) {
     // may grow concurrently; its elements do not move).
     Meta h = carve_meta(s.h_meta, cfg.max_batch_rows);
     for (size_t i = 0; i < take.size(); ++i) {
-      const Request& r = reqs[take[i]];
+      const Request& r = req(take[i]);
       h.ptrs[i] = r.vals;
       h.lens[i] = r.len;
       h.ks[i] = r.k;
@@ -253,8 +270,10 @@ void rk_server::finish(Slot& s, bool ok) {
   const double t = now_s();
   std::lock_guard<std::mutex> lk(mu);
   for (uint64_t tk : s.tickets) {
-    reqs[tk].state = ok ? RQ_DONE : RQ_FAILED;
-    reqs[tk].t_done = t;
+    Request& r = req(tk);
+    r.state = ok ? RQ_DONE : RQ_FAILED;
+    r.t_done = t;
+    if (ok) latencies_us.push_back((t - r.t_arrive) * 1e6);
   }
   if (!ok) failed += (int64_t)s.tickets.size();
   inflight -= s.bytes;
@@ -322,8 +341,8 @@ void rk_server::loop() {
       if (error != RK_OK && in_flight == 0) {
         // after a HIP error: fail what is queued and admit nothing more
         for (uint64_t tk : pending) {
-          reqs[tk].state = RQ_FAILED;
-          reqs[tk].t_done = now_s();
+          req(tk).state = RQ_FAILED;
+          req(tk).t_done = now_s();
         }
         failed += (int64_t)pending.size();
         pending.clear();
@@ -424,8 +443,14 @@ int rk_server_create(const rk_server_config* cfg, rk_server** out) {
   return RK_OK;
 }
 
-int rk_server_submit(rk_server* s, const void* vals, int64_t len, int32_t k, void* out_vals,
-                     int32_t* out_idx, uint64_t* ticket) {
+}  // extern "C"
+
+namespace {
+
+// Queues one request arriving at t_arrive (<= now): the replay passes the
+// scheduled arrival, so a late submission does not hide queueing delay.
+int submit_at(rk_server* s, const void* vals, int64_t len, int32_t k, void* out_vals,
+              int32_t* out_idx, uint64_t* ticket, double t_arrive) {
   if (!s || !ticket || len < 0 || k < 0 || (!vals && len > 0) ||
       (k > 0 && (!out_vals || !out_idx)))
     return RK_E_ARG;
@@ -440,22 +465,35 @@ int rk_server_submit(rk_server* s, const void* vals, int64_t len, int32_t k, voi
   r.bytes = (uint64_t)len * s->elem + (uint64_t)k * (s->elem + sizeof(int32_t));
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->error != RK_OK) return s->error;
-  r.t_arrive = now_s();
-  *ticket = s->reqs.size();
+  r.t_arrive = t_arrive;
+  *ticket = s->base + s->reqs.size();
   s->reqs.push_back(r);
   s->pending.push_back(*ticket);
   s->cv_work.notify_one();
   return RK_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int rk_server_submit(rk_server* s, const void* vals, int64_t len, int32_t k, void* out_vals,
+                     int32_t* out_idx, uint64_t* ticket) {
+  return submit_at(s, vals, len, k, out_vals, out_idx, ticket, now_s());
+}
+
 int rk_server_wait(rk_server* s, uint64_t ticket, double* latency_us) {
   if (!s) return RK_E_ARG;
   std::unique_lock<std::mutex> lk(s->mu);
-  if (ticket >= s->reqs.size()) return RK_E_ARG;
-  s->cv_done.wait(lk, [&] { return s->reqs[ticket].state >= RQ_DONE; });
-  const Request& r = s->reqs[ticket];
+  // tickets already reported (retired) or never issued
+  if (ticket < s->base || ticket >= s->base + s->reqs.size()) return RK_E_ARG;
+  s->cv_done.wait(lk, [&] { return s->req(ticket).state >= RQ_DONE; });
+  const Request& r = s->req(ticket);
+  if (r.reported) return RK_E_ARG;
   if (latency_us) *latency_us = (r.t_done - r.t_arrive) * 1e6;
-  switch (r.state) {
+  const int state = r.state;
+  s->reported(ticket);
+  switch (state) {
     case RQ_DONE: return RK_OK;
     case RQ_REJECTED: return RK_E_WORKSPACE;
     default: return s->error != RK_OK ? s->error : RK_E_DEVICE;
@@ -481,45 +519,34 @@ int rk_server_replay(rk_server* s, int32_t n, const void* const* vals, const int
     for (double t = now_s(); t < due; t = now_s()) {
       if (due - t > 3e-4) std::this_thread::sleep_for(std::chrono::microseconds(100));
     }
-    const int rc =
-        rk_server_submit(s, vals[i], lens[i], ks[i], out_vals[i], out_idx[i], &tickets[i]);
+    // open-loop load: latency counts from the scheduled arrival
+    const int rc = submit_at(s, vals[i], lens[i], ks[i], out_vals[i], out_idx[i], &tickets[i],
+                             std::min(now_s(), due));
     if (rc != RK_OK) {
       (void)rk_server_drain(s);
       return rc;
     }
-    std::lock_guard<std::mutex> lk(s->mu);
-    // open-loop load: latency counts from the scheduled arrival, so a late
-    // submission does not hide queueing delay
-    s->reqs[tickets[i]].t_arrive = std::min(s->reqs[tickets[i]].t_arrive, due);
   }
   int rc = rk_server_drain(s);
   std::lock_guard<std::mutex> lk(s->mu);
   double last = t0;
   for (int32_t i = 0; i < n; ++i) {
-    const Request& r = s->reqs[tickets[i]];
+    const Request& r = s->req(tickets[i]);
     if (latency_us) latency_us[i] = r.state == RQ_DONE ? (r.t_done - r.t_arrive) * 1e6 : -1.0;
     if (r.state == RQ_FAILED && rc == RK_OK) rc = RK_E_DEVICE;
     last = std::max(last, r.t_done);
   }
+  for (int32_t i = 0; i < n; ++i) s->reported(tickets[i]);
   if (makespan_s) *makespan_s = last - t0;
   return rc;
 }
 
 int rk_server_get_stats(rk_server* s, rk_server_stats* out) {
   if (!s || !out) return RK_E_ARG;
-  std::lock_guard<std::mutex> lk(s->mu);
   std::vector<double> lat;
-  lat.reserve(s->reqs.size());
-  for (const Request& r : s->reqs)
-    if (r.state == RQ_DONE) lat.push_back((r.t_done - r.t_arrive) * 1e6);
-  std::sort(lat.begin(), lat.end());
-  auto pct = [&](double q) {  // nearest-rank percentile
-    if (lat.empty()) return 0.0;
-    const size_t rank = (size_t)std::ceil(q * (double)lat.size());
-    return lat[std::min(lat.size(), std::max<size_t>(rank, 1)) - 1];
-  };
+  std::unique_lock<std::mutex> lk(s->mu);
+  lat = s->latencies_us;  // a copy under the lock; sorted without it
   std::memset(out, 0, sizeof(*out));
-  out->completed = (int64_t)lat.size();
   out->rejected = s->rejected;
   out->failed = s->failed;
   out->batches = s->batches;
@@ -527,14 +554,23 @@ int rk_server_get_stats(rk_server* s, rk_server_stats* out) {
   out->server_bytes = (int64_t)s->server_bytes;
   out->peak_inflight_bytes = (int64_t)s->peak_inflight;
   out->mean_batch_rows = s->batches ? (double)s->batched_rows / (double)s->batches : 0.0;
+  out->max_batch_rows = s->max_rows_seen;
+  out->issue_us = s->issues ? s->issue_s / (double)s->issues * 1e6 : 0.0;
+  out->retained = (int64_t)s->reqs.size();
+  lk.unlock();
+  std::sort(lat.begin(), lat.end());
+  auto pct = [&](double q) {  // nearest-rank percentile
+    if (lat.empty()) return 0.0;
+    const size_t rank = (size_t)std::ceil(q * (double)lat.size());
+    return lat[std::min(lat.size(), std::max<size_t>(rank, 1)) - 1];
+  };
+  out->completed = (int64_t)lat.size();
   out->p50_us = pct(0.50);
   out->p99_us = pct(0.99);
   double sum = 0.0;
   for (double v : lat) sum += v;
   out->mean_us = lat.empty() ? 0.0 : sum / (double)lat.size();
   out->max_us = lat.empty() ? 0.0 : lat.back();
-  out->max_batch_rows = s->max_rows_seen;
-  out->issue_us = s->issues ? s->issue_s / (double)s->issues * 1e6 : 0.0;
   return RK_OK;
 }
 

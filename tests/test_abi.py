@@ -173,4 +173,4 @@ def test_server_argument_validation():
     assert L.rk_server_drain(None) == rk.RK_E_ARG
     assert L.rk_server_get_stats(None, None) == rk.RK_E_ARG
     assert L.rk_server_destroy(None) == rk.RK_E_ARG
-    assert ctypes.sizeof(rk._ServerConfig) == 40 and ctypes.sizeof(rk._ServerStats) == 112
+    assert ctypes.sizeof(rk._ServerConfig) == 40 and ctypes.sizeof(rk._ServerStats) == 120

@@ -10,7 +10,8 @@ against a per-query threshold inside the score kernel, then selected exactly.
   lower index), sizes that are not multiples of the tiles.
 - The second pass: a database whose sampled rows hold the best scores makes
   the first threshold admit fewer than k candidates.
-- Candidate overflow (every score tied) is reported in the status word.
+- Candidate overflow (every score tied) is reported in the status word, and
+  the failed queries' outputs are filled with NaN / -1.
 - hipGraph capture: replays equal the eager result, also after the queries
   change in place.
 """
@@ -133,9 +134,11 @@ def test_all_tied_scores_report_overflow():
     n, k, d = 100_000, 10, 32
     x = torch.zeros(n, d, dtype=torch.bfloat16, device="cuda")
     q = torch.ones(3, d, dtype=torch.bfloat16, device="cuda")
-    _, _, ws = rk.knn_topk(q, x, k)
+    v, i, ws = rk.knn_topk(q, x, k)
     st = rk.status(ws)
     assert st[0] & 4 and st[1] == 3, st  # candidates outgrew the slots
+    # the failed queries read as failed: value NaN, index -1
+    assert torch.isnan(v).all() and bool((i == -1).all())
 
 
 def test_arguments_rejected():

@@ -180,3 +180,31 @@ def test_smallest_bf16_server():
         assert srv.stats()["failed"] == 0
     np.testing.assert_array_equal(oi.cpu().numpy(), ei)
     np.testing.assert_array_equal(from_device_bits(ov, "bf16"), eb)
+
+
+def test_waited_requests_are_released():
+    """The server releases a request once its outcome is reported: after
+    submit + wait cycles nothing is retained, a second wait on a ticket is
+    RK_E_ARG, and the statistics still cover every completed request."""
+    import torch
+
+    import radik_amd as rk
+    n, rows = 1 << 16, 300
+    vals = torch.randn(n, device="cuda")
+    ov = torch.empty(rows * 16, device="cuda")
+    oi = torch.empty(rows * 16, dtype=torch.int32, device="cuda")
+    with rk.Server(max_k=16, max_batch_rows=8, max_batch_keys=1 << 20, streams=2) as srv:
+        tickets = []
+        for i in range(rows):
+            tickets.append(srv.submit(vals[: 1000 + i * 100], 16, ov[16 * i:], oi[16 * i:]))
+            if i % 3 == 2:
+                for t in tickets:
+                    srv.wait(t)
+                tickets = []
+        for t in tickets:
+            srv.wait(t)
+        st = srv.stats()
+        assert st["completed"] == rows and st["retained"] == 0, st
+        assert srv.wait_code(5) == rk.RK_E_ARG
+    ref = torch.topk(vals[: 1000 + 299 * 100], 16).indices.to(torch.int32)
+    assert torch.equal(oi[16 * 299:16 * 300], ref)
