@@ -122,6 +122,10 @@ def main() -> int:
                     help="skip the single-row and quantile sweeps")
     ap.add_argument("--sweeps-only", action="store_true",
                     help="run only the sweeps (prints their JSON object)")
+    ap.add_argument("--no-fallback", dest="fallback", action="store_false",
+                    help="skip the fallback-cost measurement (sampler-defeating rows)")
+    ap.add_argument("--fallback-only", action="store_true",
+                    help="run only the fallback-cost measurement (prints its JSON object)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -147,6 +151,9 @@ def main() -> int:
     if args.sweeps_only:
         print(json.dumps({"sweeps": sweeps(args)}), flush=True)
         return 0
+    if args.fallback_only:
+        print(json.dumps({"fallback_cost": fallback_cost(args)}), flush=True)
+        return 0
     w = make_workload(args.workload)
     result = measure(w, args, primary=True)
     if result.get("bit_exact") is False:
@@ -164,6 +171,9 @@ def main() -> int:
             return 1
     if args.sweeps:
         result["sweeps"] = sweeps(args)
+        torch.cuda.empty_cache()
+    if args.fallback:
+        result["fallback_cost"] = fallback_cost(args)
         torch.cuda.empty_cache()
 
     if args.serve_requests > 0:
@@ -573,6 +583,87 @@ def sweeps(args) -> dict:
                         "ordered_rows"],
             "points": quant},
     }
+
+
+def fallback_cost(args) -> dict:
+    """Cost of the sampled path's fallback: 64 rows x 2^22 fp32, k = 1024,
+    largest, sorted, default tunables, as N(0,1) rows (control) and as rows
+    built against the sample (W.adversarial: "hidden" overflows the candidate
+    slots, "decoy" leaves fewer than k keys above the threshold), so that
+    every row takes the exact passes after its sampled attempt.  Also the
+    exact path (RK_EXACT) on the control rows.  Each point: hipGraph replays
+    (mean of 10), fallback rows from the device status, and the output
+    checked against the oracle."""
+    import numpy as np
+    import torch
+
+    import radik_amd as rk
+    import radik_amd.workloads as W
+    from oracle import oracle as orc
+
+    L = rk.lib()
+    gstream = torch.cuda.Stream()
+    sp = gstream.cuda_stream
+    saved = rk.get_tune()
+    rk.set_tune(tile=0, grid=0, sample_groups=0, group_elems=0, hist_bits=0, unroll=0)
+    rows, n, k = 64, 1 << 22, 1024
+    flags = rk.RK_LARGEST | rk.RK_SORTED
+    out = {"rows": rows, "len": n, "k": k, "dtype": "f32", "tunables": "defaults",
+           "timing": "hipGraph replays, mean of 10",
+           "columns": ["ms", "keys_per_s", "roofline_frac", "fallback_rows", "bit_exact"]}
+    for kind in ("control", "hidden", "decoy", "control_exact"):
+        w = (W.adversarial(kind, rows=rows, n=n, k=k) if kind in ("hidden", "decoy") else
+             W.Workload(kind, np.full(rows, n, np.int64), np.full(rows, k, np.int32),
+                        True, 6, [W.Segment(0, rows, W.NORMAL, 0.0, 1.0)]))
+        fl = flags | (rk.RK_EXACT if kind == "control_exact" else 0)
+        vals = W.materialize(w)
+        off = w.offsets()
+        d_off = torch.from_numpy(off).cuda()
+        d_k = torch.from_numpy(w.ks).cuda()
+        d_oo = torch.from_numpy(w.out_offsets()).cuda()
+        ws_bytes = rk.workspace_size(rk.RK_F32, rows, w.total, k, fl)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+        ov = torch.empty(rows * k, dtype=torch.float32, device="cuda")
+        oi = torch.empty(rows * k, dtype=torch.int32, device="cuda")
+
+        def call():
+            rc = L.rk_topk_batched(rk.RK_F32, vals.data_ptr(), d_off.data_ptr(), d_k.data_ptr(),
+                                   rows, w.total, k, fl, None, ov.data_ptr(), oi.data_ptr(),
+                                   d_oo.data_ptr(), ws.data_ptr(), ws_bytes, sp)
+            if rc != 0:
+                raise rk.RadikError(rc, "rk_topk_batched (fallback cost)")
+
+        torch.cuda.synchronize()
+        with rk.Graph.capture(gstream) as g:
+            call()
+        for _ in range(2):
+            g.launch(gstream)
+        gstream.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(gstream)
+        for _ in range(10):
+            g.launch(gstream)
+        e1.record(gstream)
+        gstream.synchronize()
+        g.close()
+        ms = e0.elapsed_time(e1) / 10
+        st = rk.status(ws, gstream)
+        bits = vals.view(torch.int32).cpu().numpy().view(np.uint32)
+        eb, ei = orc.topk(bits, off, w.ks, flags=orc.LARGEST | orc.SORTED,
+                          nthreads=orc.usable_threads())
+        exact = bool(st[0] == 0 and np.array_equal(oi.cpu().numpy(), ei) and
+                     np.array_equal(ov.view(torch.int32).cpu().numpy().view(np.uint32), eb))
+        alg = w.algorithmic_bytes()
+        out[kind] = [round(ms, 4), w.total / (ms / 1e3),
+                     round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4), int(st[1]), exact]
+        log(f"fallback cost {kind}: {ms:.3f} ms, fallback rows {st[1]}, bit-exact {exact}")
+        del vals, ws, ov, oi, bits
+        torch.cuda.empty_cache()
+    rk.set_tune(**saved)
+    out["hidden_vs_control"] = round(out["hidden"][0] / out["control"][0], 3)
+    out["decoy_vs_control"] = round(out["decoy"][0] / out["control"][0], 3)
+    return out
 
 
 def knn_bench(args) -> dict:

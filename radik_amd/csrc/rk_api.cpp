@@ -61,7 +61,7 @@ size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct Layout {
   size_t hdr, desc, st, tile_start, cand_base, cand_cnt, order, ghist, tile_cnt, chunk_row,
-      cand, total;
+      cand, xrows, xst, total;
   uint64_t tile_cap, cand_cap;
 };
 
@@ -89,6 +89,8 @@ Layout make_layout(int32_t R, int64_t total_len, int32_t max_k) {
   L.tile_cnt = o; o = align256(o + L.tile_cap * sizeof(uint2));
   L.chunk_row = o; o = align256(o + L.tile_cap * sizeof(uint32_t));
   L.cand = o; o = align256(o + L.cand_cap * sizeof(uint64_t));
+  L.xrows = o; o = align256(o + Ru * sizeof(uint32_t));
+  L.xst = o; o = align256(o + Ru * sizeof(rk::XRow));
   L.total = o;
   return L;
 }
@@ -190,6 +192,8 @@ int launch(int dtype, const void* vals, const int64_t* row_off, const void* cons
   p.tile_cnt = reinterpret_cast<uint2*>(ws + L.tile_cnt);
   p.chunk_row = reinterpret_cast<uint32_t*>(ws + L.chunk_row);
   p.cand = reinterpret_cast<uint64_t*>(ws + L.cand);
+  p.xrows = reinterpret_cast<uint32_t*>(ws + L.xrows);
+  p.xst = reinterpret_cast<rk::XRow*>(ws + L.xst);
   p.tile_capacity = L.tile_cap;
   p.cand_capacity = L.cand_cap;
   // LDS of the select kernels sized to this call: the k selected keys (and
@@ -219,6 +223,8 @@ int launch(int dtype, const void* vals, const int64_t* row_off, const void* cons
   int64_t ct = std::max<int64_t>(1, chunk_bytes / ((int64_t)p.tune.tile * es));
   while (ct > 1 && (int64_t)tiles_bound / ct < 2 * (int64_t)grid) ct >>= 1;
   p.chunk_tiles = (int32_t)ct;
+  // The exact-pass executor hands rows out in chunks of at least 64K keys.
+  p.x_chunk_tiles = std::max(1, 65536 / p.tune.tile);
   const hipError_t e =
       rk::run_pipeline(dtype, p, static_cast<hipStream_t>(stream), row_grid, grid, part, publish);
   return e == hipSuccess ? RK_OK : RK_E_HIP;

@@ -200,6 +200,26 @@ RK_DEV Vec<DT> unpack(const uint4& x) {
 }
 
 // ---------------------------------------------------------------------------
+// Device-coherent accesses between the workgroups of one running kernel.
+// The XCDs' L2 caches are not coherent with each other for plain accesses;
+// agent-scope atomic loads and stores (and read-modify-writes) are, without
+// the L2 write-back / invalidation of an agent-scope fence.  A workgroup
+// that signals another one with an atomic first waits for its own earlier
+// accesses to complete (wait_vmem), so they are performed before the signal.
+// ---------------------------------------------------------------------------
+template <typename T>
+RK_DEV T ld_dev(const T* a) {
+  return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+RK_DEV void st_dev(T* a, T v) {
+  __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// This wave's outstanding vector memory operations (loads, stores, atomics)
+// completed.
+RK_DEV void wait_vmem() { __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// ---------------------------------------------------------------------------
 // Wave64 / workgroup primitives.
 // ---------------------------------------------------------------------------
 RK_DEV int lane_id() { return threadIdx.x & 63; }

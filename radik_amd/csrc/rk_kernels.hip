@@ -414,6 +414,8 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
     h->sel_done = 0;
     h->q_chunk[0] = 0;
     h->q_chunk[1] = 0;
+    h->x_rows = 0;
+    h->x_left = 0;
     if (carry_t > p.tile_capacity) status |= ST_TILES;
     if (carry_c > p.cand_capacity) status |= ST_CANDIDATES;
     h->status = status;
@@ -483,11 +485,13 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
 }
 
 // Sends row r to the exact passes from its first digit: histogram cleared,
-// nothing resolved, all of its k still to take.  Called by a whole workgroup
-// of NT threads of the kernel that makes the decision, so the fallback round
-// needs no initialisation launch of its own.
+// nothing resolved, all of its k still to take, and the row added to the
+// exact-pass executor's list.  Called by a whole workgroup of NT threads of
+// the kernel that makes the decision (after every thread has read the row's
+// state), so the fallback round needs no initialisation launch of its own.
 template <int NT>
 RK_DEV void exact_start_row(const KParams& p, int r, uint32_t why) {
+  __syncthreads();  // the row's state is read before it changes
   uint32_t* h = p.ghist + (size_t)r * HIST_MAX_BINS;
   for (int i = threadIdx.x; i < HIST_MAX_BINS; i += NT) h[i] = 0;
   if (threadIdx.x == 0) {
@@ -499,6 +503,23 @@ RK_DEV void exact_start_row(const KParams& p, int r, uint32_t why) {
     w.why = why;
     w.phase = PH_HIST;
     p.cand_cnt[r] = 0;
+    // chunks of up to x_chunk_tiles tiles, but at least 64 per row (when it
+    // has that many tiles), so that a single row still spreads over the grid
+    const uint32_t tiles = p.tile_start[r + 1] - p.tile_start[r];
+    uint32_t ct = tiles / 64;
+    ct = ct < 1 ? 1 : (ct > (uint32_t)p.x_chunk_tiles ? (uint32_t)p.x_chunk_tiles : ct);
+    XRow& x = p.xst[r];
+    x.claim = 0;
+    x.done = 0;
+    x.ct = ct;
+    x.nch = (tiles + ct - 1) / ct;
+    const uint32_t slot = atomicAdd(&p.hdr->x_rows, 1u);
+    if (slot < (uint32_t)p.R) {  // a row enters at most once per call
+      p.xrows[slot] = (uint32_t)r;
+      atomicAdd(&p.hdr->x_left, 1u);
+    } else {
+      atomicOr(&p.hdr->status, (uint32_t)ST_INTERNAL);
+    }
   }
 }
 
@@ -627,8 +648,10 @@ struct AppendCtx {
   uint32_t cap;        // candidate slots of the row
 };
 
-// Moves the wave's staged keys to the row's candidate slots.
-RK_DEV void wave_flush(const AppendCtx& ac, uint32_t& n) {
+// Moves the wave's staged keys to the row's candidate slots.  `full` is set
+// (wave-uniform) once the row's candidates exceed its slots: the row then
+// takes the exact passes, and the wave stops streaming it.
+RK_DEV void wave_flush(const AppendCtx& ac, uint32_t& n, bool& full) {
   if (n == 0) return;
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   uint32_t gb = 0;
@@ -637,6 +660,7 @@ RK_DEV void wave_flush(const AppendCtx& ac, uint32_t& n) {
   for (uint32_t i = lane_id(); i < n; i += 64)
     if (gb + i < ac.cap) ac.cand[gb + i] = ac.stage[i];
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  if (gb + n > ac.cap) full = true;
   n = 0;
 }
 
@@ -645,7 +669,8 @@ RK_DEV void wave_flush(const AppendCtx& ac, uint32_t& n) {
 // one ballot per element position: position c of lane l goes to
 //   sum over c' < c of popcount(ballot_c') + popcount(ballot_c below lane l).
 template <int DT, bool LARGEST>
-RK_DEV void append_vec(const AppendCtx& ac, uint32_t& n, uint4 x, uint32_t mask, int64_t first) {
+RK_DEV void append_vec(const AppendCtx& ac, uint32_t& n, bool& full, uint4 x, uint32_t mask,
+                       int64_t first) {
   constexpr int P = Traits<DT>::PACK;
   const Vec<DT> v = unpack<DT>(x);
   uint64_t bal[P];
@@ -656,7 +681,7 @@ RK_DEV void append_vec(const AppendCtx& ac, uint32_t& n, uint4 x, uint32_t mask,
     tot += (uint32_t)__popcll(bal[c]);
   }
   if (tot == 0) return;
-  if (n + tot > (uint32_t)WSTAGE) wave_flush(ac, n);  // tot <= 64 * P <= WSTAGE
+  if (n + tot > (uint32_t)WSTAGE) wave_flush(ac, n, full);  // tot <= 64 * P <= WSTAGE
   uint32_t base = n;
 #pragma unroll
   for (int c = 0; c < P; ++c) {
@@ -670,33 +695,35 @@ RK_DEV void append_vec(const AppendCtx& ac, uint32_t& n, uint4 x, uint32_t mask,
   n += tot;
 }
 
-// At most 72 VGPRs (7 waves per SIMD) for 2 loads in flight and 80 (6
-// waves) for 4, so that the rarely taken candidate path costs no occupancy
-// and no spills.
+// LDS of one streaming workgroup's candidate staging.
+struct FilterLds {
+  uint64_t stage[STREAM_NT / 64][WSTAGE];
+  uint4 vstage[STREAM_NT / 64][VSTAGE];
+  uint32_t vpos[STREAM_NT / 64][VSTAGE];
+  uint32_t flush_n[STREAM_NT / 64], flush_base[STREAM_NT / 64];
+};
+
+// Tiles [ta, tb) of row r (a PH_FILTER row): every key >= T appended to the
+// row's candidate slots.  Called by a whole workgroup of STREAM_NT threads.
 template <int DT, bool LARGEST, int U>
-__global__ __launch_bounds__(STREAM_NT)
-__attribute__((amdgpu_waves_per_eu(U >= 8 ? 1 : (U == 4 ? 6 : 7))))
-void k_filter(KParams p) {
-  if (p.round && p.hdr->fallback_rows == 0) return;  // nothing fell back
+RK_DEV void filter_segment(const KParams& p, FilterLds& lds, const RowState& st, const RowDesc& d,
+                           uint64_t cbase, int r, uint32_t ta, uint32_t tb) {
   using UT = typename Traits<DT>::U;
   constexpr int P = Traits<DT>::PACK;
-  __shared__ uint64_t stage[STREAM_NT / 64][WSTAGE];
-  __shared__ uint4 vstage[STREAM_NT / 64][VSTAGE];
-  __shared__ uint32_t vpos[STREAM_NT / 64][VSTAGE];
-  __shared__ uint32_t flush_n[STREAM_NT / 64], flush_base[STREAM_NT / 64];
+  auto& stage = lds.stage;
+  auto& vstage = lds.vstage;
+  auto& vpos = lds.vpos;
+  auto& flush_n = lds.flush_n;
+  auto& flush_base = lds.flush_base;
   const int lane = lane_id();
   // wave-uniform (readfirstlane tells the compiler), so the streaming loop's
   // control flow stays scalar and its loads stay unconditional
   const int wave_base = __builtin_amdgcn_readfirstlane(threadIdx.x & ~63);
-
-  auto body = [&](int r, uint32_t ta, uint32_t tb) {
-    // the row's metadata, loaded together (one round trip)
-    const RowState st = p.st[r];
-    const RowDesc d = p.desc[r];
-    const uint64_t cbase = p.cand_base[r];
+  {
     if (st.phase != PH_FILTER) return;  // uniform across the block
     const uint32_t T = st.T;
     const uint32_t cap = st.cap;
+
     const bool all = (T == 0);
     const uint32_t xm = prefilter_mask<DT, LARGEST>(T);
     const UT* rb = static_cast<const UT*>(d.base);
@@ -727,6 +754,7 @@ void k_filter(KParams p) {
     ac.cand_cnt = &p.cand_cnt[r];
     ac.cap = cap;
     uint32_t staged = 0;  // keys in this wave's stage (wave-uniform)
+    bool full = false;    // the row's slots overflowed (wave-uniform)
     uint4* vst = vstage[wave_base >> 6];
     uint32_t* vps = vpos[wave_base >> 6];
     uint32_t vstaged = 0;  // vectors in this wave's vector stage (wave-uniform)
@@ -745,7 +773,7 @@ void k_filter(KParams p) {
         for (int c = 0; c < P; ++c) mm |= (sel_key<DT, LARGEST>(v.e[c]) >= T ? 1u : 0u) << c;
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      if (__ballot(mm != 0)) append_vec<DT, LARGEST>(ac, staged, xv, mm, (int64_t)first);
+      if (__ballot(mm != 0)) append_vec<DT, LARGEST>(ac, staged, full, xv, mm, (int64_t)first);
       vstaged = 0;
     };
 
@@ -762,13 +790,13 @@ void k_filter(KParams p) {
         x.x = u0;  // element 0 of the vector (16-bit types: low half)
         mask = pred(u0) ? 1u : 0u;
       }
-      if (__ballot(mask != 0)) append_vec<DT, LARGEST>(ac, staged, x, mask, pos);
+      if (__ballot(mask != 0)) append_vec<DT, LARGEST>(ac, staged, full, x, mask, pos);
     }
     // Aligned body.
     const uint4* vp = reinterpret_cast<const uint4*>(rb + al);
     if (all) {
       // Threshold admits every key (kept simple: such rows are rare).
-      for (int64_t jb = wave_base; jb < nv; jb += STREAM_NT) {
+      for (int64_t jb = wave_base; jb < nv && !full; jb += STREAM_NT) {
         const int64_t j = jb + lane;
         uint4 x = make_uint4(0u, 0u, 0u, 0u);
         uint32_t m = 0;
@@ -776,7 +804,7 @@ void k_filter(KParams p) {
           x = ldg16<false>(vp + j);
           m = (1u << P) - 1u;
         }
-        append_vec<DT, LARGEST>(ac, staged, x, m, al + j * P);
+        append_vec<DT, LARGEST>(ac, staged, full, x, m, al + j * P);
       }
     } else {
       // Buffer loads, U x 16 bytes per lane per step.  Ranges are cut into
@@ -828,7 +856,9 @@ void k_filter(KParams p) {
           test_set(x, jb);
 #pragma unroll
           for (int u = 0; u < U; ++u) x[u] = y[u];
+          if (full) break;
         }
+        if (full) break;
       }
     }
     flush_vecs();
@@ -851,8 +881,24 @@ void k_filter(KParams p) {
     for (uint32_t i = lane; i < staged; i += 64)
       if (fb + i < ac.cap) ac.cand[fb + i] = ac.stage[i];
     staged = 0;
-  };
-  for_each_chunk_queued(p, &p.hdr->q_chunk[p.round], body);
+  }
+}
+
+// The sampled round's streaming pass over every PH_FILTER row.  At most 72
+// VGPRs (7 waves per SIMD) for 2 loads in flight and 80 (6 waves) for 4, so
+// that the rarely taken candidate path costs no occupancy and no spills.
+template <int DT, bool LARGEST, int U>
+__global__ __launch_bounds__(STREAM_NT)
+__attribute__((amdgpu_waves_per_eu(U >= 8 ? 1 : (U == 4 ? 6 : 7))))
+void k_filter(KParams p) {
+  __shared__ FilterLds lds;
+  for_each_chunk_queued(p, &p.hdr->q_chunk[0], [&](int r, uint32_t ta, uint32_t tb) {
+    // the row's metadata, loaded together (one round trip)
+    const RowState st = p.st[r];
+    const RowDesc d = p.desc[r];
+    const uint64_t cbase = p.cand_base[r];
+    filter_segment<DT, LARGEST, U>(p, lds, st, d, cbase, r, ta, tb);
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -1278,9 +1324,19 @@ __global__ __launch_bounds__(SEL_NT) void k_merge_rank(KParams p) {
 // Exact passes (RadiK's countBin / selectBin, radixselect_l.cuh:14-126,
 // 173-267, on the selection key): per pass, a histogram of the next digit of
 // the keys that share the resolved prefix, then the bin holding the k-th
-// key.  A row leaves as soon as the keys >= the selected bin's lower bound
-// fit its candidate slots (threshold known exactly), or, with every bit
-// resolved, as PH_ORDERED.
+// key.  A row leaves the passes as soon as the keys >= the selected bin's
+// lower bound fit its candidate slots (threshold known exactly: one filter
+// read, then the select), or, with every bit resolved, as PH_ORDERED (the
+// ordered emit's counting and writing reads).
+//
+// All of it runs in one persistent launch, k_exact, over a device-side work
+// list: the rows handed to it (xrows) advance through their passes on their
+// own, each pass split into chunks of x_chunk_tiles tiles that any
+// workgroup may claim; the workgroup that completes a pass's last chunk
+// selects the bin and opens the row's next pass.  So an idle call costs one
+// launch (no row fell back), and a row that falls back streams on the whole
+// grid.  No workgroup waits on another one's progress except for chunks that
+// running workgroups hold, so the launch needs no co-residency.
 // ---------------------------------------------------------------------------
 // Exact calls (RK_EXACT) only: in sampled calls the sample and select
 // kernels start the rows they send to the exact passes (exact_start_row).
@@ -1288,169 +1344,102 @@ __global__ __launch_bounds__(256) void k_exact_init(KParams p) {
   if (p.hdr->fallback_rows == 0) return;
   for (int r = blockIdx.x; r < p.R; r += gridDim.x) {
     const RowState s = p.st[r];
-    if (s.phase != PH_HIST || !(s.flags & RF_NEED_INIT)) continue;
-    uint32_t* h = p.ghist + (size_t)r * HIST_MAX_BINS;
-    for (int i = threadIdx.x; i < HIST_MAX_BINS; i += 256) h[i] = 0;
-    if (threadIdx.x == 0) {
-      RowState& w = p.st[r];
-      w.prefix = 0;
-      w.resolved = 0;
-      w.rem = (uint32_t)p.desc[r].k;
-      w.flags = 0;
-      p.cand_cnt[r] = 0;
-    }
+    if (s.phase == PH_HIST && (s.flags & RF_NEED_INIT)) exact_start_row<256>(p, r, FB_NONE);
   }
 }
 
+// Histogram LDS of one streaming workgroup: one sub-histogram per wave.
+struct HistLds {
+  uint32_t sub[STREAM_NT / 64][HIST_MAX_BINS];
+};
+
+// Digit histogram of tiles [ta, tb) of PH_HIST row r added to the row's
+// global histogram: per-wave LDS sub-histograms (one atomic per wave when
+// all its active lanes hit one bin, as on skewed rows), merged with one
+// global atomic per non-empty bin.  Called by a whole workgroup.
 template <int DT, bool LARGEST, int U>
-__global__ __launch_bounds__(STREAM_NT) void k_hist(KParams p) {
-  if (p.hdr->fallback_rows == 0) return;
+RK_DEV void hist_segment(const KParams& p, HistLds& lds, const RowState& st, const RowDesc& d,
+                         int r, uint32_t ta, uint32_t tb) {
   using UT = typename Traits<DT>::U;
   constexpr int P = Traits<DT>::PACK;
   constexpr int KB = Traits<DT>::KEY_BITS;
   constexpr int NW = STREAM_NT / 64;
-  __shared__ uint32_t sub[NW][HIST_MAX_BINS];
-  for (int i = threadIdx.x; i < NW * HIST_MAX_BINS; i += STREAM_NT) (&sub[0][0])[i] = 0;
+  for (int i = threadIdx.x; i < NW * HIST_MAX_BINS; i += STREAM_NT) (&lds.sub[0][0])[i] = 0;
   __syncthreads();
   const int lane = lane_id();
-  const int wave = threadIdx.x >> 6;
   const int wave_base = threadIdx.x & ~63;
-  uint32_t* h = sub[wave];
+  uint32_t* h = lds.sub[threadIdx.x >> 6];
   const int HB = p.tune.hist_bits;
+  const uint32_t b = st.resolved;
+  const uint32_t P0 = st.prefix;
+  const int w = (KB - (int)b) < HB ? (KB - (int)b) : HB;
+  const int shift = KB - (int)b - w;
+  const uint32_t nb = 1u << w;
+  const UT* rb = static_cast<const UT*>(d.base);
+  const int64_t es = (int64_t)ta * p.tune.tile;
+  int64_t ee = (int64_t)tb * p.tune.tile;
+  if (ee > d.len) ee = d.len;
+  int64_t al = aligned_start<DT>(rb, es);
+  if (al > ee) al = ee;
+  const int64_t nv = (ee - al) / P;
+  const int64_t tail0 = al + nv * P;
 
-  for_each_segment(p, [&](int r, uint32_t ta, uint32_t tb) {
-    if (p.st[r].phase != PH_HIST) return;
-    const RowDesc d = p.desc[r];
-    const uint32_t b = p.st[r].resolved;
-    const uint32_t P0 = p.st[r].prefix;
-    const int w = (KB - (int)b) < HB ? (KB - (int)b) : HB;
-    const int shift = KB - (int)b - w;
-    const uint32_t nb = 1u << w;
-    const UT* rb = static_cast<const UT*>(d.base);
-    const int64_t es = (int64_t)ta * p.tune.tile;
-    int64_t ee = (int64_t)tb * p.tune.tile;
-    if (ee > d.len) ee = d.len;
-    int64_t al = aligned_start<DT>(rb, es);
-    if (al > ee) al = ee;
-    const int64_t nv = (ee - al) / P;
-    const int64_t tail0 = al + nv * P;
+  auto count = [&](uint32_t u, bool valid) {
+    const uint32_t sk = sel_key<DT, LARGEST>(u);
+    const bool ok = valid && (b == 0 || (uint32_t)((uint64_t)sk >> (KB - b)) == P0);
+    const uint32_t dg = (sk >> shift) & (nb - 1);
+    const uint64_t act = __ballot(ok);
+    if (act) {
+      const int fl = __ffsll((unsigned long long)act) - 1;
+      const uint32_t d0 = __shfl(dg, fl, 64);
+      const uint64_t same = __ballot(ok && dg == d0);
+      if (same == act) {
+        if (lane == fl) atomicAdd(&h[d0], (uint32_t)__popcll(act));
+      } else if (ok) {
+        atomicAdd(&h[dg], 1u);
+      }
+    }
+  };
 
-    auto count = [&](uint32_t u, bool valid) {
-      const uint32_t sk = sel_key<DT, LARGEST>(u);
-      const bool ok = valid && (b == 0 || (uint32_t)((uint64_t)sk >> (KB - b)) == P0);
-      const uint32_t dg = (sk >> shift) & (nb - 1);
-      const uint64_t act = __ballot(ok);
-      if (act) {
-        const int fl = __ffsll((unsigned long long)act) - 1;
-        const uint32_t d0 = __shfl(dg, fl, 64);
-        const uint64_t same = __ballot(ok && dg == d0);
-        if (same == act) {
-          if (lane == fl) atomicAdd(&h[d0], (uint32_t)__popcll(act));
-        } else if (ok) {
-          atomicAdd(&h[dg], 1u);
-        }
-      }
-    };
-
-    if (wave_base == 0) {
-      int64_t pos = -1;
-      if (lane < P && es + lane < al) pos = es + lane;
-      if (lane >= 32 && lane - 32 < P && tail0 + (lane - 32) < ee) pos = tail0 + (lane - 32);
-      const uint32_t u = pos >= 0 ? (uint32_t)rb[pos] : 0u;
-      count(u, pos >= 0);
-    }
-    const uint4* vp = reinterpret_cast<const uint4*>(rb + al);
-    for (int64_t jb = wave_base; jb < nv; jb += (int64_t)STREAM_NT * U) {
-      uint4 x[U];
+  if (wave_base == 0) {
+    int64_t pos = -1;
+    if (lane < P && es + lane < al) pos = es + lane;
+    if (lane >= 32 && lane - 32 < P && tail0 + (lane - 32) < ee) pos = tail0 + (lane - 32);
+    const uint32_t u = pos >= 0 ? (uint32_t)rb[pos] : 0u;
+    count(u, pos >= 0);
+  }
+  const uint4* vp = reinterpret_cast<const uint4*>(rb + al);
+  for (int64_t jb = wave_base; jb < nv; jb += (int64_t)STREAM_NT * U) {
+    uint4 x[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int64_t j = jb + lane + (int64_t)u * STREAM_NT;
-        x[u] = j < nv ? ldg16<false>(vp + j) : make_uint4(0u, 0u, 0u, 0u);
-      }
+    for (int u = 0; u < U; ++u) {
+      const int64_t j = jb + lane + (int64_t)u * STREAM_NT;
+      x[u] = j < nv ? ldg16<false>(vp + j) : make_uint4(0u, 0u, 0u, 0u);
+    }
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int64_t j = jb + lane + (int64_t)u * STREAM_NT;
-        const Vec<DT> v = unpack<DT>(x[u]);
+    for (int u = 0; u < U; ++u) {
+      const int64_t j = jb + lane + (int64_t)u * STREAM_NT;
+      const Vec<DT> v = unpack<DT>(x[u]);
 #pragma unroll
-        for (int c = 0; c < P; ++c) count(v.e[c], j < nv);
-      }
+      for (int c = 0; c < P; ++c) count(v.e[c], j < nv);
     }
-    __syncthreads();
-    uint32_t* g = p.ghist + (size_t)r * HIST_MAX_BINS;
-    for (uint32_t i = threadIdx.x; i < nb; i += STREAM_NT) {
-      uint32_t s = 0;
+  }
+  __syncthreads();
+  uint32_t* g = p.ghist + (size_t)r * HIST_MAX_BINS;
+  for (uint32_t i = threadIdx.x; i < nb; i += STREAM_NT) {
+    uint32_t s = 0;
 #pragma unroll
-      for (int q = 0; q < NW; ++q) {
-        s += sub[q][i];
-        sub[q][i] = 0;
-      }
-      if (s) atomicAdd(&g[i], s);
-    }
-    __syncthreads();
-  });
-}
-
-__global__ __launch_bounds__(256) void k_hist_select(KParams p, int key_bits) {
-  if (p.hdr->fallback_rows == 0) return;
-  __shared__ uint32_t hist[HIST_MAX_BINS];
-  __shared__ uint32_t scratch[16];
-  __shared__ uint32_t outv[4];
-  const int KB = key_bits;
-  const int HB = p.tune.hist_bits;
-  for (int r = blockIdx.x; r < p.R; r += gridDim.x) {
-    const RowState s = p.st[r];
-    if (s.phase != PH_HIST) continue;
-    const int b = (int)s.resolved;
-    const int w = (KB - b) < HB ? (KB - b) : HB;
-    const uint32_t nb = 1u << w;
-    uint32_t* g = p.ghist + (size_t)r * HIST_MAX_BINS;
-    for (uint32_t i = threadIdx.x; i < nb; i += 256) {
-      hist[i] = g[i];
-      g[i] = 0;
-    }
-    __syncthreads();
-    block_find_crossing<256>(hist, (int)nb, s.rem, scratch, outv);
-    if (threadIdx.x == 0) {
-      RowState& st = p.st[r];
-      const uint32_t bin = outv[0];
-      if (bin == 0xffffffffu) {
-        atomicOr(&p.hdr->status, (uint32_t)ST_INTERNAL);
-        st.phase = PH_DONE;
-      } else {
-        const uint32_t prefix = (s.prefix << w) | bin;
-        const uint32_t resolved = (uint32_t)(b + w);
-        const uint32_t rem = s.rem - outv[1];
-        const uint32_t k = (uint32_t)p.desc[r].k;
-        const uint64_t need = (uint64_t)(k - rem) + outv[2];
-        if (need <= s.cap) {
-          st.T = prefix << (KB - (int)resolved);
-          st.phase = PH_FILTER;
-          st.flags = RF_EXACT_T;
-          p.cand_cnt[r] = 0;
-        } else if ((int)resolved == KB) {
-          st.V = prefix;
-          st.rem = rem;
-          st.phase = PH_ORDERED;
-          atomicAdd(&p.hdr->ordered_rows, 1u);
-        } else {
-          st.prefix = prefix;
-          st.resolved = resolved;
-          st.rem = rem;
-        }
-      }
-    }
-    __syncthreads();
+    for (int q = 0; q < NW; ++q) s += lds.sub[q][i];
+    if (s) atomicAdd(&g[i], s);
   }
 }
 
-// ---------------------------------------------------------------------------
 // Ordered emit for rows whose ties at the k-th key V exceed the candidate
-// slots (e.g. an all-equal row).  Per tile, counts of keys > V and == V;
-// then each tile writes (its prefix summed over the row's earlier tiles), in
-// row-position order,
-// its keys > V and those == V whose rank among equals is below rem.  The
-// output position of a selected element is gt_before + min(rem, eq_before).
-// ---------------------------------------------------------------------------
+// slots (e.g. an all-equal row).  Counting pass: per chunk, the keys > V
+// and == V; the pass's last workgroup turns the chunk counts into prefix
+// sums.  Writing pass: each chunk writes, in row-position order, its keys
+// > V and those == V whose rank among equals is below rem, at output
+// position gt_before + min(rem, eq_before).
 template <int DT>
 RK_DEV uint32_t load_slot(const typename Traits<DT>::U* rb, int64_t es, int64_t al, int64_t nv,
                           int64_t ee, int64_t s, Vec<DT>& v, int64_t& first) {
@@ -1480,122 +1469,333 @@ RK_DEV uint32_t load_slot(const typename Traits<DT>::U* rb, int64_t es, int64_t 
 }
 
 template <int DT, bool LARGEST>
-__global__ __launch_bounds__(STREAM_NT) void k_emit_count(KParams p) {
-  if (p.hdr->ordered_rows == 0) return;
+RK_DEV void emit_count_chunk(const KParams& p, const RowState& st, const RowDesc& d, int r,
+                             uint32_t c, uint32_t ta, uint32_t tb, uint32_t* scratch) {
   using UT = typename Traits<DT>::U;
   constexpr int P = Traits<DT>::PACK;
-  __shared__ uint32_t scratch[16];
-  for_each_segment(p, [&](int r, uint32_t ta, uint32_t tb) {
-    if (p.st[r].phase != PH_ORDERED) return;
-    const RowDesc d = p.desc[r];
-    const uint32_t V = p.st[r].V;
-    const UT* rb = static_cast<const UT*>(d.base);
-    for (uint32_t t = ta; t < tb; ++t) {
-      const int64_t es = (int64_t)t * p.tune.tile;
-      int64_t ee = es + p.tune.tile;
-      if (ee > d.len) ee = d.len;
-      int64_t al = aligned_start<DT>(rb, es);
-      if (al > ee) al = ee;
-      const int64_t nv = (ee - al) / P;
-      uint32_t gt = 0, eq = 0;
-      for (int64_t s = threadIdx.x; s < nv + 2; s += STREAM_NT) {
-        Vec<DT> v;
-        int64_t first;
-        const uint32_t mask = load_slot<DT>(rb, es, al, nv, ee, s, v, first);
+  const uint32_t V = st.V;
+  const UT* rb = static_cast<const UT*>(d.base);
+  uint32_t gt = 0, eq = 0;
+  for (uint32_t t = ta; t < tb; ++t) {
+    const int64_t es = (int64_t)t * p.tune.tile;
+    int64_t ee = es + p.tune.tile;
+    if (ee > d.len) ee = d.len;
+    int64_t al = aligned_start<DT>(rb, es);
+    if (al > ee) al = ee;
+    const int64_t nv = (ee - al) / P;
+    for (int64_t s = threadIdx.x; s < nv + 2; s += STREAM_NT) {
+      Vec<DT> v;
+      int64_t first;
+      const uint32_t mask = load_slot<DT>(rb, es, al, nv, ee, s, v, first);
 #pragma unroll
-        for (int c = 0; c < P; ++c) {
-          if ((mask >> c) & 1u) {
-            const uint32_t sk = sel_key<DT, LARGEST>(v.e[c]);
-            gt += sk > V ? 1u : 0u;
-            eq += sk == V ? 1u : 0u;
-          }
+      for (int e = 0; e < P; ++e) {
+        if ((mask >> e) & 1u) {
+          const uint32_t sk = sel_key<DT, LARGEST>(v.e[e]);
+          gt += sk > V ? 1u : 0u;
+          eq += sk == V ? 1u : 0u;
         }
       }
-      gt = block_sum<STREAM_NT>(gt, scratch);
-      eq = block_sum<STREAM_NT>(eq, scratch);
-      if (threadIdx.x == 0) p.tile_cnt[p.tile_start[r] + t] = make_uint2(gt, eq);
     }
-  });
+  }
+  gt = block_sum<STREAM_NT>(gt, scratch);
+  eq = block_sum<STREAM_NT>(eq, scratch);
+  if (threadIdx.x == 0)
+    st_dev(reinterpret_cast<uint64_t*>(p.tile_cnt + p.tile_start[r] + c),
+           ((uint64_t)eq << 32) | gt);
 }
 
 template <int DT, bool LARGEST>
-__global__ __launch_bounds__(STREAM_NT) void k_emit_write(KParams p) {
-  if (p.hdr->ordered_rows == 0) return;
+RK_DEV void emit_write_chunk(const KParams& p, const RowState& st, const RowDesc& d, int r,
+                             uint32_t c, uint32_t ta, uint32_t tb, uint32_t* scratch) {
   using UT = typename Traits<DT>::U;
   constexpr int P = Traits<DT>::PACK;
-  __shared__ uint32_t scratch[16];
-  for_each_segment(p, [&](int r, uint32_t ta, uint32_t tb) {
-    if (p.st[r].phase != PH_ORDERED) return;
-    const RowDesc d = p.desc[r];
-    const uint32_t V = p.st[r].V;
-    const uint32_t rem = p.st[r].rem;
-    const uint32_t kk = (uint32_t)d.k;
-    const UT* rb = static_cast<const UT*>(d.base);
-    UT* ov = static_cast<UT*>(d.out_v);
-    // Keys > V and == V in the row's tiles before this segment (the counts
-    // k_emit_count wrote); within the segment the running totals carry on.
-    const uint2* tc = p.tile_cnt + p.tile_start[r];
-    uint32_t pg = 0, pe = 0;
-    for (uint32_t t = threadIdx.x; t < ta; t += STREAM_NT) {
-      const uint2 c = tc[t];
-      pg += c.x;
-      pe += c.y;
-    }
-    uint32_t run_g = block_sum<STREAM_NT>(pg, scratch);
-    uint32_t run_e = block_sum<STREAM_NT>(pe, scratch);
-    for (uint32_t t = ta; t < tb; ++t) {
-      const int64_t es = (int64_t)t * p.tune.tile;
-      int64_t ee = es + p.tune.tile;
-      if (ee > d.len) ee = d.len;
-      int64_t al = aligned_start<DT>(rb, es);
-      if (al > ee) al = ee;
-      const int64_t nv = (ee - al) / P;
-      for (int64_t s0 = 0; s0 < nv + 2; s0 += STREAM_NT) {
-        const int64_t s = s0 + threadIdx.x;
-        Vec<DT> v;
-        int64_t first = 0;
-        uint32_t mask = 0;
-        if (s < nv + 2) mask = load_slot<DT>(rb, es, al, nv, ee, s, v, first);
-        uint32_t g = 0, q = 0;
-        uint32_t sk[P];
+  const uint32_t V = st.V;
+  const uint32_t rem = st.rem;
+  const uint32_t kk = (uint32_t)d.k;
+  const UT* rb = static_cast<const UT*>(d.base);
+  UT* ov = static_cast<UT*>(d.out_v);
+  // keys > V and == V in the row's chunks before this one
+  const uint64_t pre = ld_dev(reinterpret_cast<const uint64_t*>(p.tile_cnt + p.tile_start[r] + c));
+  uint32_t run_g = (uint32_t)pre, run_e = (uint32_t)(pre >> 32);
+  for (uint32_t t = ta; t < tb; ++t) {
+    const int64_t es = (int64_t)t * p.tune.tile;
+    int64_t ee = es + p.tune.tile;
+    if (ee > d.len) ee = d.len;
+    int64_t al = aligned_start<DT>(rb, es);
+    if (al > ee) al = ee;
+    const int64_t nv = (ee - al) / P;
+    for (int64_t s0 = 0; s0 < nv + 2; s0 += STREAM_NT) {
+      const int64_t s = s0 + threadIdx.x;
+      Vec<DT> v;
+      int64_t first = 0;
+      uint32_t mask = 0;
+      if (s < nv + 2) mask = load_slot<DT>(rb, es, al, nv, ee, s, v, first);
+      uint32_t g = 0, q = 0;
+      uint32_t sk[P];
 #pragma unroll
-        for (int c = 0; c < P; ++c) {
-          sk[c] = ((mask >> c) & 1u) ? sel_key<DT, LARGEST>(v.e[c]) : 0u;
-          const bool ok = (mask >> c) & 1u;
-          g += (ok && sk[c] > V) ? 1u : 0u;
-          q += (ok && sk[c] == V) ? 1u : 0u;
-        }
-        uint32_t tot;
-        const uint32_t ex = block_excl_scan<STREAM_NT>(g | (q << 16), scratch, tot);
-        uint32_t gb = run_g + (ex & 0xffffu);
-        uint32_t qb = run_e + (ex >> 16);
+      for (int e = 0; e < P; ++e) {
+        sk[e] = ((mask >> e) & 1u) ? sel_key<DT, LARGEST>(v.e[e]) : 0u;
+        const bool ok = (mask >> e) & 1u;
+        g += (ok && sk[e] > V) ? 1u : 0u;
+        q += (ok && sk[e] == V) ? 1u : 0u;
+      }
+      uint32_t tot;
+      const uint32_t ex = block_excl_scan<STREAM_NT>(g | (q << 16), scratch, tot);
+      uint32_t gb = run_g + (ex & 0xffffu);
+      uint32_t qb = run_e + (ex >> 16);
 #pragma unroll
-        for (int c = 0; c < P; ++c) {
-          if (!((mask >> c) & 1u)) continue;
-          const uint32_t pos = (uint32_t)(first + c);
-          if (sk[c] > V) {
-            const uint32_t o = gb + (qb < rem ? qb : rem);
-            if (o < kk) {
-              ov[o] = (UT)v.e[c];
-              put_idx(p, d, o, pos);  // a position: mapped by k_finalize_ordered
-            }
-            ++gb;
-          } else if (sk[c] == V) {
-            if (qb < rem) {
-              const uint32_t o = gb + qb;
-              if (o < kk) {
-                ov[o] = (UT)v.e[c];
-                put_idx(p, d, o, pos);
-              }
-            }
-            ++qb;
+      for (int e = 0; e < P; ++e) {
+        if (!((mask >> e) & 1u)) continue;
+        const uint32_t pos = (uint32_t)(first + e);
+        if (sk[e] > V) {
+          const uint32_t o = gb + (qb < rem ? qb : rem);
+          if (o < kk) {
+            ov[o] = (UT)v.e[e];
+            put_idx(p, d, o, pos);  // a position: mapped by k_finalize_ordered
           }
+          ++gb;
+        } else if (sk[e] == V) {
+          if (qb < rem) {
+            const uint32_t o = gb + qb;
+            if (o < kk) {
+              ov[o] = (UT)v.e[e];
+              put_idx(p, d, o, pos);
+            }
+          }
+          ++qb;
         }
-        run_g += tot & 0xffffu;
-        run_e += tot >> 16;
+      }
+      run_g += tot & 0xffffu;
+      run_e += tot >> 16;
+    }
+  }
+}
+
+// A histogram pass of row r is complete: the bin of the k-th key (the
+// global histogram is cleared as it is read, for the next pass), then the
+// row's next stage.  Returns (to every thread) whether the row leaves the
+// executor: to the select (threshold known) or on an internal error.
+template <int KB>
+RK_DEV bool finish_hist(const KParams& p, int r, const RowState& s, uint32_t* hist,
+                        uint32_t* scratch, uint32_t* outv) {
+  const int HB = p.tune.hist_bits;
+  const int b = (int)s.resolved;
+  const int w = (KB - b) < HB ? (KB - b) : HB;
+  const uint32_t nb = 1u << w;
+  uint32_t* g = p.ghist + (size_t)r * HIST_MAX_BINS;
+  for (uint32_t i = threadIdx.x; i < nb; i += STREAM_NT) {
+    hist[i] = ld_dev(g + i);
+    st_dev(g + i, 0u);
+  }
+  __syncthreads();
+  block_find_crossing<STREAM_NT>(hist, (int)nb, s.rem, scratch, outv);
+  if (threadIdx.x == 0) {
+    RowState& st = p.st[r];
+    const uint32_t bin = outv[0];
+    uint32_t leave = 0;
+    if (bin == 0xffffffffu) {
+      atomicOr(&p.hdr->status, (uint32_t)ST_INTERNAL);
+      st_dev(&st.phase, (uint32_t)PH_DONE);
+      leave = 1;
+    } else {
+      const uint32_t prefix = (s.prefix << w) | bin;
+      const uint32_t resolved = (uint32_t)(b + w);
+      const uint32_t rem = s.rem - outv[1];
+      const uint32_t k = (uint32_t)p.desc[r].k;
+      const uint64_t need = (uint64_t)(k - rem) + outv[2];
+      if (need <= s.cap) {
+        // the filter pass follows here, then the row leaves for the select
+        st_dev(&st.T, prefix << (KB - (int)resolved));
+        st_dev(&st.phase, (uint32_t)PH_FILTER);
+        st_dev(&st.flags, (uint32_t)RF_EXACT_T);
+        st_dev(p.cand_cnt + r, 0u);
+      } else if ((int)resolved == KB) {
+        st_dev(&st.V, prefix);
+        st_dev(&st.rem, rem);
+        st_dev(&st.phase, (uint32_t)PH_ORDERED);
+        st_dev(&st.flags, 0u);
+        atomicAdd(&p.hdr->ordered_rows, 1u);
+      } else {
+        st_dev(&st.prefix, prefix);
+        st_dev(&st.resolved, resolved);
+        st_dev(&st.rem, rem);
       }
     }
-  });
+    outv[3] = leave;
+  }
+  __syncthreads();
+  return outv[3] != 0;
+}
+
+// The counting pass of an ordered row is complete: chunk counts -> their
+// exclusive prefix sums, in place.
+RK_DEV void finish_count(const KParams& p, int r, uint32_t nch, uint32_t* scratch) {
+  uint64_t* tc = reinterpret_cast<uint64_t*>(p.tile_cnt + p.tile_start[r]);
+  uint32_t carry_g = 0, carry_e = 0;
+  for (uint32_t b0 = 0; b0 < nch; b0 += STREAM_NT) {
+    const uint32_t i = b0 + threadIdx.x;
+    const uint64_t v = i < nch ? ld_dev(tc + i) : 0ull;
+    uint32_t tg, te;
+    const uint32_t eg = block_excl_scan<STREAM_NT>((uint32_t)v, scratch, tg);
+    const uint32_t ee = block_excl_scan<STREAM_NT>((uint32_t)(v >> 32), scratch, te);
+    if (i < nch) st_dev(tc + i, ((uint64_t)(carry_e + ee) << 32) | (carry_g + eg));
+    carry_g += tg;
+    carry_e += te;
+  }
+  if (threadIdx.x == 0) st_dev(&p.st[r].flags, (uint32_t)RF_EMIT_WRITE);
+}
+
+// Longest a workgroup waits for claimable work before it reports an
+// internal error and leaves (wall clock, 100 MHz): no legitimate wait comes
+// near it (a chunk's pass plus a bin selection take microseconds).
+constexpr uint64_t X_MAX_WAIT = 100000000ull;  // 1 s
+
+// LDS (a 32 KB union of the histogram and candidate-staging buffers) allows
+// 4 workgroups per CU: registers are bounded to match.
+template <int DT, bool LARGEST, int U>
+__global__ __launch_bounds__(STREAM_NT) __attribute__((amdgpu_waves_per_eu(U >= 8 ? 1 : 4)))
+void k_exact(KParams p) {
+  const uint32_t nrows = min(p.hdr->x_rows, (uint32_t)p.R);  // set by earlier launches
+  if (nrows == 0) return;
+  constexpr int KB = Traits<DT>::KEY_BITS;
+  __shared__ union {
+    FilterLds f;
+    HistLds h;
+  } lds;
+  __shared__ uint32_t scratch[16];
+  __shared__ uint32_t outv[4];
+  __shared__ int r_sh;
+  __shared__ uint32_t c_sh, seq_sh, last_sh;
+  const int tid = threadIdx.x;
+  uint32_t cursor = blockIdx.x % nrows;
+  uint64_t wait0 = 0;
+  for (;;) {
+    // Claim a chunk: wave 0 looks at 64 rows of the list at a time, from
+    // this workgroup's cursor, and claims in the first row with chunks left.
+    if (tid < 64) {
+      int got = -1;
+      uint32_t gc = 0, gseq = 0;
+      for (uint32_t base = 0; base < nrows && got < 0; base += 64) {
+        const uint32_t i = base + (uint32_t)tid;
+        int r = -1;
+        bool avail = false;
+        if (i < nrows) {
+          r = (int)p.xrows[(cursor + i) % nrows];
+          const uint64_t w = __hip_atomic_load(&p.xst[r].claim, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+          avail = (uint32_t)(w >> 32) != XSEQ_DONE && (uint32_t)w < p.xst[r].nch;
+        }
+        uint64_t m = __ballot(avail);
+        while (m) {
+          const int l = __ffsll((unsigned long long)m) - 1;
+          m &= m - 1;
+          uint64_t old = 0;
+          if (tid == l)
+            old = __hip_atomic_fetch_add(&p.xst[r].claim, 1ull, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+          old = __shfl(old, l, 64);
+          const int rl = __shfl(r, l, 64);
+          if ((uint32_t)(old >> 32) != XSEQ_DONE && (uint32_t)old < p.xst[rl].nch) {
+            got = rl;
+            gc = (uint32_t)old;
+            gseq = (uint32_t)(old >> 32);
+            cursor = (cursor + base + (uint32_t)l) % nrows;
+            break;
+          }
+        }
+      }
+      if (tid == 0) {
+        r_sh = got;
+        c_sh = gc;
+        seq_sh = gseq;
+      }
+    }
+    __syncthreads();
+    const int r = r_sh;
+    const uint32_t c = c_sh, seq = seq_sh;
+    __syncthreads();
+    if (r < 0) {
+      // Nothing claimable: done when every row has left, else wait for the
+      // workgroups that hold the last chunks of a pass to open the next one.
+      const uint32_t left = __hip_atomic_load(&p.hdr->x_left, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+      if (left == 0) return;
+      const uint64_t now = wall_clock64();
+      if (wait0 == 0) wait0 = now;
+      if (now - wait0 > X_MAX_WAIT) {
+        if (tid == 0) atomicOr(&p.hdr->status, (uint32_t)ST_INTERNAL);
+        return;
+      }
+      __builtin_amdgcn_s_sleep(8);
+      continue;
+    }
+    wait0 = 0;
+    // The row's pass state, as the workgroup that opened this pass left it
+    // (read after the claim that returned the pass's sequence number).
+    RowState st;
+    {
+      const uint32_t* src = reinterpret_cast<const uint32_t*>(p.st + r);
+      uint32_t* dst = reinterpret_cast<uint32_t*>(&st);
+#pragma unroll
+      for (int i = 0; i < (int)(sizeof(RowState) / 4); ++i) dst[i] = ld_dev(src + i);
+    }
+    const RowDesc d = p.desc[r];  // constant in this launch: plain loads
+    const uint32_t nch = p.xst[r].nch, ct = p.xst[r].ct;
+    const uint32_t tiles = p.tile_start[r + 1] - p.tile_start[r];
+    const uint32_t ta = c * ct;
+    const uint32_t tb = tiles - ta < ct ? tiles : ta + ct;
+    if (st.phase == PH_HIST) {
+      hist_segment<DT, LARGEST, U>(p, lds.h, st, d, r, ta, tb);
+    } else if (st.phase == PH_FILTER) {
+      filter_segment<DT, LARGEST, U>(p, lds.f, st, d, p.cand_base[r], r, ta, tb);
+    } else if (st.phase == PH_ORDERED && !(st.flags & RF_EMIT_WRITE)) {
+      emit_count_chunk<DT, LARGEST>(p, st, d, r, c, ta, tb, scratch);
+    } else if (st.phase == PH_ORDERED) {
+      emit_write_chunk<DT, LARGEST>(p, st, d, r, c, ta, tb, scratch);
+    } else if (tid == 0) {
+      atomicOr(&p.hdr->status, (uint32_t)ST_INTERNAL);  // a claimed chunk in no pass
+    }
+    // Chunk done: this workgroup's histogram atomics and chunk counts
+    // (device-coherent) are performed before the count that may let another
+    // workgroup finish the pass.  (Candidates and outputs are read by later
+    // launches only.)
+    wait_vmem();
+    __syncthreads();
+    if (tid == 0) {
+      const uint32_t prev = __hip_atomic_fetch_add(&p.xst[r].done, 1u, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+      last_sh = prev + 1 == nch ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!last_sh) continue;
+    // This workgroup completed the pass: the row's next stage.
+    bool leave;
+    if (st.phase == PH_HIST) {
+      leave = finish_hist<KB>(p, r, st, lds.h.sub[0], scratch, outv);
+    } else if (st.phase == PH_ORDERED && !(st.flags & RF_EMIT_WRITE)) {
+      finish_count(p, r, nch, scratch);
+      leave = false;
+    } else {
+      leave = true;  // filter pass done (the select follows) / ordered rows written
+    }
+    // The next pass's state (device-coherent stores) is performed before the
+    // claim word that opens it.
+    wait_vmem();
+    __syncthreads();
+    if (tid == 0) {
+      if (leave) {
+        st_dev(&p.xst[r].claim, (uint64_t)XSEQ_DONE << 32);
+        wait_vmem();
+        __hip_atomic_fetch_add(&p.hdr->x_left, 0xffffffffu, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        st_dev(&p.xst[r].done, 0u);
+        wait_vmem();
+        st_dev(&p.xst[r].claim, (uint64_t)(seq + 1) << 32);
+      }
+    }
+    __syncthreads();
+  }
 }
 
 template <int DT, bool LARGEST>
@@ -1731,37 +1931,30 @@ static int device_cus() {
 
 template <int DT, bool LARGEST, int U>
 static hipError_t run_typed_u(const KParams& p, const Launch& L0) {
-  constexpr int KB = Traits<DT>::KEY_BITS;
   const size_t dyn = (size_t)p.lds_keys * sizeof(uint64_t);
   const size_t dyn_sample = (size_t)p.sample_keys * sizeof(uint32_t);
   const int rows_blocks = (p.R + 255) / 256;
   hipStream_t s = L0.stream;
   // Persistent streaming kernels: every workgroup resident from the start.
   const int fgrid = std::min(L0.grid, resident_grid(k_filter<DT, LARGEST, U>, STREAM_NT));
-  const int hgrid = std::min(L0.grid, resident_grid(k_hist<DT, LARGEST, U>, STREAM_NT));
+  const int xgrid = std::min(L0.grid, resident_grid(k_exact<DT, LARGEST, U>, STREAM_NT));
   // Per-row kernels loop over rows: one resident wave of workgroups suffices.
   auto rows = [&](int resident) { return std::max(1, std::min(L0.row_grid, resident)); };
   const int g_sample = rows(resident_grid(k_sample<DT, LARGEST>, SEL_NT, dyn_sample));
   const int g_select = rows(resident_grid(k_select_finalize<DT, LARGEST>, SEL_NT, dyn));
   const int g_init = rows(resident_grid(k_exact_init, 256));
-  int g_hsel = rows(resident_grid(k_hist_select, 256));
   int g_ford = rows(resident_grid(k_finalize_ordered<DT, LARGEST>, SEL_NT, dyn));
   KParams p1 = p;  // fallback round: kernels return at once when no row fell back
   p1.round = 1;
   // In a sampled call the fallback round is almost always idle, and an idle
-  // launch costs about as much as dispatching its workgroups: one workgroup
-  // per CU each (measured: 13 idle launches of full persistent grids took
-  // ~4.5 us each on 512-row calls, ~1.5 us at one workgroup per CU).  Rows
-  // that do fall back still stream at 256 workgroups.  Exact calls keep the
-  // full grids (their passes are the work).
+  // launch costs about as much as dispatching its workgroups: its per-row
+  // kernels run one workgroup per CU.  The exact-pass executor keeps its
+  // full grid, so that rows that do fall back stream at full bandwidth.
   const bool exact_call = (p.flags & FL_EXACT) != 0;
-  int fb_f = fgrid, fb_h = hgrid, fb_sel = g_select;
+  int fb_sel = g_select;
   if (!exact_call) {
     const int cus = device_cus();
-    fb_f = std::min(fb_f, cus);
-    fb_h = std::min(fb_h, cus);
     fb_sel = std::min(fb_sel, cus);
-    g_hsel = std::min(g_hsel, cus);
     g_ford = std::min(g_ford, cus);
   }
   if (L0.part != PART_FALLBACK) {
@@ -1797,17 +1990,10 @@ static hipError_t run_typed_u(const KParams& p, const Launch& L0) {
   // rows entering the exact passes are initialised by the kernel that sends
   // them there; only an exact call starts them all here
   if (exact_call) k_exact_init<<<g_init, 256, 0, s>>>(p1);
-  const int passes = (KB + p.tune.hist_bits - 1) / p.tune.hist_bits;
-  for (int i = 0; i < passes; ++i) {
-    k_hist<DT, LARGEST, U><<<fb_h, STREAM_NT, 0, s>>>(p1);
-    k_hist_select<<<g_hsel, 256, 0, s>>>(p1, KB);
-  }
-  k_filter<DT, LARGEST, U><<<fb_f, STREAM_NT, 0, s>>>(p1);
+  k_exact<DT, LARGEST, U><<<xgrid, STREAM_NT, 0, s>>>(p1);
   // COPYALL / SMALL rows of an exact-path call are finalized here too.
   KParams pf = exact_call ? p : p1;
   k_select_finalize<DT, LARGEST><<<fb_sel, SEL_NT, dyn, s>>>(pf);
-  k_emit_count<DT, LARGEST><<<fb_f, STREAM_NT, 0, s>>>(p1);
-  k_emit_write<DT, LARGEST><<<fb_f, STREAM_NT, 0, s>>>(p1);
   k_finalize_ordered<DT, LARGEST><<<g_ford, SEL_NT, dyn, s>>>(p1);
   return hipGetLastError();
 }

@@ -28,6 +28,7 @@ constexpr int MERGE_CHUNK = 4096;
 enum RowFlag : uint32_t {
   RF_NEED_INIT = 1u,  // PH_HIST row whose histogram/state must be reset
   RF_EXACT_T = 2u,    // T came from exact histograms: the select cannot fail
+  RF_EMIT_WRITE = 4u, // PH_ORDERED row past its counting pass: writing
 };
 
 enum Status : uint32_t {
@@ -59,6 +60,20 @@ struct RowState {
   uint32_t why;       // why the row left the sampled path (diagnostics, Fallback)
 };
 
+// Per-row work state of the exact-pass executor (k_exact).  A row's passes
+// (digit histograms, then the filter or the ordered emit's two passes) are
+// stages of nch chunks each; `claim` hands the chunks of the current stage
+// out and carries the stage's sequence number, so a workgroup that claims a
+// chunk knows which stage it belongs to.
+struct XRow {
+  uint64_t claim;  // (stage sequence << 32) | next chunk of the stage
+  uint32_t done;   // chunks of the current stage completed
+  uint32_t nch;    // chunks per stage
+  uint32_t ct;     // tiles per chunk
+  uint32_t pad;
+};
+constexpr uint32_t XSEQ_DONE = 0xffffffffu;  // claim's sequence once the row has left
+
 enum Fallback : uint32_t {
   FB_NONE = 0,
   FB_SAMPLE_DEGENERATE = 1,  // threshold would admit every key (NaN-heavy / rank >= sample)
@@ -77,7 +92,9 @@ struct Header {
   uint32_t q_select[2];    //   (select: sampled round, fallback round)
   uint32_t sel_done;       // workgroups of the sampled-round select that finished
   uint32_t q_chunk[2];     // chunk queue heads of the filter (sampled, fallback round)
-  uint32_t pad[52];
+  uint32_t x_rows;         // rows handed to the exact-pass executor (xrows entries)
+  uint32_t x_left;         // of those, rows whose passes are not finished
+  uint32_t pad[50];
 };
 static_assert(sizeof(Header) == 256, "workspace header is 256 bytes");
 
@@ -123,6 +140,8 @@ struct KParams {
   uint2* tile_cnt;       // [tile_capacity] ordered emit: keys > V, == V per tile
   uint32_t* chunk_row;   // [tile_capacity] row of each filter chunk, or ~0u
   uint64_t* cand;        // [cand_capacity]
+  uint32_t* xrows;       // [R] rows of the exact-pass executor, in arrival order
+  XRow* xst;             // [R] their work state
   uint64_t tile_capacity;
   uint64_t cand_capacity;
   int32_t lds_keys;     // LDS key capacity of the select kernels (this launch)
@@ -132,6 +151,7 @@ struct KParams {
   int32_t chunk_tiles;  // tiles per chunk of the interleaved filter partition
   int32_t merge;        // select: sort rows with k > MERGE_CHUNK in chunks
                         // (the merge kernels follow this select launch)
+  int32_t x_chunk_tiles;  // most tiles per chunk of the exact-pass executor
   Tune tune;
   // When set (sampled-round select only): the last select workgroup to finish
   // copies the header's first 8 words here (mapped host memory).
@@ -151,9 +171,9 @@ constexpr int TILE_MIN = 1024;
 
 // Which part of the launch sequence a call issues.  PART_MAIN is the sampled
 // round (plan, sample, filter, select_finalize); PART_FALLBACK the exact
-// passes and ordered emit that only do work when the header's fallback_rows
-// is non-zero.  A caller that reads the header between the two (the serving
-// loop) skips the idle second half; PART_ALL issues both back to back.
+// passes and ordered emit that only do work when rows fell back.  A caller
+// that reads the header between the two (the serving loop) skips the idle
+// second half; PART_ALL issues both back to back.
 enum Part : int { PART_ALL = 0, PART_MAIN = 1, PART_FALLBACK = 2 };
 
 }  // namespace rk

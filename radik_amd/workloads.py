@@ -11,7 +11,9 @@ Shapes follow the five configurations the top-k path is measured on:
 * ``c4``: 2048 rows, len = floor(2^U(10, 24)), k = floor(2^U(0, log2 min(len,
   4096))), N(0,1), fp32, or bf16 / f16 rounded from the same fp32 (seed 4);
 * ``c5``: serving requests, len 2^U(12, 22), k 2^U(0, 12), a mixture of
-  uniform / normal / Zipf / near-constant rows, Poisson arrivals (seed 5).
+  uniform / normal / Zipf / near-constant rows, Poisson arrivals (seed 5);
+* ``adversarial``: equal rows built against the sampled path's sample
+  (seed 6), so that every row takes the exact passes.
 
 Row metadata is drawn on the host from a counter-based SplitMix64 stream;
 values are generated on the device by ``rk_gen_fill`` (same construction).
@@ -64,6 +66,7 @@ class Workload:
     seed: int
     segments: list = field(default_factory=list)
     dtype: str = "f32"        # "f32" | "bf16" | "f16"
+    adversary: str = ""       # "" | "hidden" | "decoy" (see adversarial())
 
     @property
     def rows(self) -> int:
@@ -131,6 +134,62 @@ def c4(dtype: str = "f32", scale: int = 1, rows: int = 2048) -> Workload:
     ks = np.maximum(ks, 1)
     return Workload(f"c4_batched_uneven_{dtype}", lens, ks, True, 4,
                     [Segment(0, rows, NORMAL, 0.0, 1.0)], dtype)
+
+
+def sample_positions(n: int, sample_groups: int = 1024, group_elems: int = 16) -> np.ndarray:
+    """Row positions the sampled path reads for a row of n elements: runs of
+    group_elems keys starting at floor(g * (n - group_elems) / (ng - 1)), with
+    ng the number of runs the library derives from sample_groups and n (at
+    most 16384 keys, about 1/64 of the row, at least 64 runs).  Used to build
+    inputs that defeat the sample; the sampled path itself is exact whatever
+    the sample sees."""
+    ge = group_elems
+    keys = min(16384, max(sample_groups, 64) * ge)
+    ng = sample_groups
+    if ng * ge * 64 > n:
+        ng = n // (64 * ge)
+    ng = max(ng, min(n // (2 * ge), 64))
+    ng = max(min(ng, keys // ge), 1)
+    step = (n - ge) / (ng - 1) if ng > 1 else 0.0
+    starts = np.minimum((np.arange(ng) * step).astype(np.int64), n - ge)
+    return (starts[:, None] + np.arange(ge)[None, :]).ravel()
+
+
+def adversarial(kind: str, rows: int = 64, n: int = 1 << 22, k: int = 1024, dtype: str = "f32",
+                largest: bool = True) -> Workload:
+    """Rows built against the sample of the sampled path (seed 6), N(0,1)
+    values shifted by 8 in the direction of the selection at chosen positions:
+
+    * ``hidden``: every position except the sampled ones (for 512 and 1024
+      runs of 16 keys) is shifted: the sample sees none of the best keys, its
+      threshold admits nearly the whole row and the candidates overflow;
+    * ``decoy``: only the sampled positions (1024 runs of 16 keys, the default
+      tunables) are shifted: the sample sees only the best keys, and fewer
+      than k keys of the row reach its threshold.
+
+    Every row falls back to the exact passes; the output is still exact."""
+    if kind not in ("hidden", "decoy"):
+        raise ValueError(f"unknown adversary {kind!r}")
+    return Workload(f"adversarial_{kind}_{dtype}", np.full(rows, n, np.int64),
+                    np.full(rows, min(k, n), np.int32), largest, 6,
+                    [Segment(0, rows, NORMAL, 0.0, 1.0)], dtype, kind)
+
+
+def _apply_adversary(w: Workload, vals, off) -> None:
+    import torch
+
+    shift = 8.0 if w.largest else -8.0
+    for n in np.unique(w.lens):
+        if w.adversary == "decoy":
+            pos = sample_positions(int(n), 1024, 16)
+        else:
+            seen = np.union1d(sample_positions(int(n), 1024, 16), sample_positions(int(n), 512, 16))
+            pos = np.setdiff1d(np.arange(int(n)), seen)
+        d_pos = torch.from_numpy(pos).to(vals.device)
+        for r in np.nonzero(w.lens == n)[0]:
+            row = vals[int(off[r]):int(off[r + 1])]
+            row.index_add_(0, d_pos, torch.full((len(pos),), shift, dtype=vals.dtype,
+                                                 device=vals.device))
 
 
 def c5_requests(n: int = 100_000, scale: int = 1):
@@ -204,8 +263,8 @@ def materialize(w: Workload, device="cuda"):
     vals = torch.empty(max(w.total, 1), dtype=tdt, device=device)
     if w.dtype != "f32":
         # 16-bit runs round the same fp32 values (round to nearest even)
-        f32 = materialize(Workload(w.name, w.lens, w.ks, w.largest, w.seed, w.segments, "f32"),
-                          device)
+        f32 = materialize(Workload(w.name, w.lens, w.ks, w.largest, w.seed, w.segments, "f32",
+                                   w.adversary), device)
         vals.copy_(f32[: vals.numel()])
         del f32
         return vals
@@ -215,4 +274,6 @@ def materialize(w: Workload, device="cuda"):
             if hi > lo:
                 rk.gen_fill(vals[lo:hi], seg.kind, w.seed, r, a=seg.a, b=seg.b,
                             dup_frac=seg.dup, row_len=hi - lo)
+    if w.adversary:
+        _apply_adversary(w, vals, off)
     return vals

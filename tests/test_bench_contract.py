@@ -45,3 +45,22 @@ def test_percentile_is_nearest_rank():
     assert bench.percentile(x, 1.00) == 100.0
     assert bench.percentile([7.0], 0.99) == 7.0
     assert bench.percentile([], 0.5) == 0.0
+
+
+def test_sample_positions_cover_evenly_spaced_runs():
+    """The adversarial workloads' picture of the sample: ng runs of 16 keys,
+    the first at 0, the last ending at the row's end, spaced about
+    (n - 16) / (ng - 1) apart, at most 16384 keys and about 1/64 of a row."""
+    sys.path.insert(0, ROOT)
+    import radik_amd.workloads as W
+    for n, sg, ng in ((1 << 20, 1024, 1024), (1 << 22, 512, 512), (1 << 22, 2048, 1024),
+                      (1 << 16, 1024, 64), (5000, 1024, 64)):
+        p = W.sample_positions(n, sg, 16)
+        assert len(p) == ng * 16
+        runs = p.reshape(ng, 16)
+        assert (np.diff(runs, axis=1) == 1).all()
+        assert runs[0, 0] == 0 and runs[-1, -1] == n - 1
+        gaps = np.diff(runs[:, 0])
+        assert gaps.min() >= (n - 16) // (ng - 1) and gaps.max() <= (n - 16) // (ng - 1) + 1
+    w = W.adversarial("hidden", rows=2, n=1 << 20)
+    assert w.adversary == "hidden" and w.rows == 2 and int(w.ks[0]) == 1024
