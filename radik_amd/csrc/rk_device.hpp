@@ -611,6 +611,105 @@ RK_DEV void block_bitonic_desc(uint64_t* a, uint32_t P2) {
   else bitonic_sort_lds<NT>(a, P2);
 }
 
+// Descending sort of m distinct 64-bit keys a[0, m), m <= NT * E, E keys
+// per thread in registers: one most-significant-digit pass and ranks within
+// buckets.  The 11 bits below the common prefix of the largest and the
+// smallest key pick one of 2048 buckets (bucket order is key order); the
+// keys are grouped by bucket, and a key's place in its bucket is the number
+// of the bucket's keys above it.  Work O(m + sum of squared bucket sizes),
+// seven barriers, where a bitonic network does O(m log^2 m) work in
+// log^2 m / 2 dependent stages.  Returns false, keys untouched, when a
+// bucket holds more than 64 keys (skewed or tied keys): the caller sorts
+// them another way.  bins: 2048 words of LDS; scratch64: 2 NT / 64 words.
+template <int NT, int E>
+RK_DEV bool bucket_sort_desc(uint64_t* a, uint32_t m, uint32_t* bins, uint64_t* scratch64,
+                             uint32_t* scratch) {
+  constexpr int NB = 2048, PER = NB / NT, NW = NT / 64;
+  static_assert(NB % NT == 0, "buckets per thread");
+  const uint32_t t = threadIdx.x;
+  // Branch-free per key (selects, no divergent writes of the register
+  // arrays, which would make the compiler move them to scratch memory).
+  uint64_t v[E];
+  uint64_t mx = 0, mn = ~0ull;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const uint32_t i = t + (uint32_t)e * NT;
+    const bool in = i < m;
+    const uint64_t x = a[in ? i : 0u];
+    v[e] = in ? x : 0ull;
+    mx = (in && x > mx) ? x : mx;
+    mn = (in && x < mn) ? x : mn;
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    const uint64_t x = __shfl_xor(mx, d, 64), y = __shfl_xor(mn, d, 64);
+    mx = x > mx ? x : mx;
+    mn = y < mn ? y : mn;
+  }
+  if ((t & 63) == 0) {
+    scratch64[t >> 6] = mx;
+    scratch64[NW + (t >> 6)] = mn;
+  }
+  for (uint32_t i = t; i < (uint32_t)NB; i += NT) bins[i] = 0;
+  __syncthreads();
+  mx = 0;
+  mn = ~0ull;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    mx = scratch64[w] > mx ? scratch64[w] : mx;
+    mn = scratch64[NW + w] < mn ? scratch64[NW + w] : mn;
+  }
+  const int L = mx == mn ? 63 : __builtin_clzll(mx ^ mn);  // common high bits
+  const int sh = 64 - L - 11 > 0 ? 64 - L - 11 : 0;
+  auto bucket = [&](uint64_t x) { return (uint32_t)(x >> sh) & (NB - 1); };
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    if (t + (uint32_t)e * NT < m) atomicAdd(&bins[bucket(v[e])], 1u);
+  __syncthreads();
+  // Bucket starts in descending bucket order: thread t scans buckets
+  // NB-1-t*PER down to NB-PER-t*PER; the largest bucket decides the method.
+  uint32_t c[PER], tot = 0, big = 0;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    c[j] = bins[NB - 1 - (t * PER + j)];
+    tot += c[j];
+    big = c[j] > big ? c[j] : big;
+  }
+  big = block_sum<NT>(big > 64 ? 1u : 0u, scratch);
+  if (big != 0) return false;  // uniform
+  uint32_t all;
+  uint32_t run = block_excl_scan<NT>(tot, scratch, all);
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    bins[NB - 1 - (t * PER + j)] = run;  // start: the bucket's cursor
+    run += c[j];
+  }
+  __syncthreads();
+  // grouped by bucket (any order inside one)
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    if (t + (uint32_t)e * NT < m) a[atomicAdd(&bins[bucket(v[e])], 1u)] = v[e];
+  __syncthreads();
+  // bins[b] is now the end of bucket b, which bucket b - 1 starts at
+  uint32_t o[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const bool in = t + (uint32_t)e * NT < m;
+    const uint32_t bk = bucket(v[e]);
+    const uint32_t st = bk == NB - 1 ? 0u : bins[bk + 1];
+    const uint32_t en = in ? bins[bk] : st;  // no keys to rank against
+    uint32_t r = st;
+    for (uint32_t j = st; j < en; ++j) r += a[j] > v[e] ? 1u : 0u;
+    o[e] = r;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    if (t + (uint32_t)e * NT < m) a[o[e]] = v[e];
+  __syncthreads();
+  return true;
+}
+
 RK_DEV uint32_t next_pow2(uint32_t x) {
   return x <= 1 ? 1u : (1u << (32 - __builtin_clz(x - 1)));
 }

@@ -543,10 +543,10 @@ __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
   for (;;) {
     if (threadIdx.x == 0) next_sh = atomicAdd(&p.hdr->q_sample, 1u);
     __syncthreads();
-    const uint32_t qi = next_sh;
+    const uint32_t qi = __builtin_amdgcn_readfirstlane(next_sh);  // wave-uniform
     __syncthreads();
     if (qi >= (uint32_t)p.R) break;
-    const int r = (int)p.order[p.R + qi];
+    const int r = __builtin_amdgcn_readfirstlane((int)p.order[p.R + qi]);
     const RowState st0 = p.st[r];
     if (st0.phase == PH_HIST && (st0.flags & RF_NEED_INIT)) {
       exact_start_row<SEL_NT>(p, r, st0.why);  // planned straight into the exact passes
@@ -951,15 +951,36 @@ RK_DEV uint32_t wave_slot(uint32_t* cnt, bool take) {
   return (uint32_t)__builtin_amdgcn_readlane((int)base, fl) + mbcnt(m);
 }
 
-// Sort sbuf[0, m) (re-keyed) and write the row's output plus its tail fill.
-template <int DT, bool LARGEST>
-RK_DEV void sort_and_write(const KParams& p, const RowDesc& d, uint64_t* sbuf, uint32_t m, int mode,
-                           int r = 0) {
-  using U = typename Traits<DT>::U;
+// LDS a sort borrows from its kernel: 2048 bucket counters and reduction
+// scratch.
+struct SortLds {
+  uint32_t* bins;
+  uint64_t* s64;  // >= 2 * SEL_NT / 64 words
+  uint32_t* s32;  // >= SEL_NT / 64 words
+};
+
+// Sorts the distinct keys sbuf[0, m) descending: by buckets and ranks when
+// they spread (bucket_sort_desc), else by a bitonic network (zeros pad it to
+// a power of two).
+RK_DEV void sort_desc(uint64_t* sbuf, uint32_t m, const SortLds& sl) {
+  __syncthreads();  // sbuf complete
+  if (m > 64 && m <= 4u * SEL_NT) {
+    const bool ok = m <= 2u * SEL_NT ? bucket_sort_desc<SEL_NT, 2>(sbuf, m, sl.bins, sl.s64, sl.s32)
+                                     : bucket_sort_desc<SEL_NT, 4>(sbuf, m, sl.bins, sl.s64, sl.s32);
+    if (ok) return;
+  }
   const uint32_t P2 = next_pow2(m);
   for (uint32_t i = m + threadIdx.x; i < P2; i += SEL_NT) sbuf[i] = 0;
   __syncthreads();
   if (P2 > 1) block_bitonic_desc<SEL_NT>(sbuf, P2);
+}
+
+// Sort sbuf[0, m) (re-keyed) and write the row's output plus its tail fill.
+template <int DT, bool LARGEST>
+RK_DEV void sort_and_write(const KParams& p, const RowDesc& d, uint64_t* sbuf, uint32_t m, int mode,
+                           const SortLds& sl, int r = 0) {
+  using U = typename Traits<DT>::U;
+  sort_desc(sbuf, m, sl);
   RK_STAMP(p, r, 13);
   (void)r;
   const U* rb = static_cast<const U*>(d.base);
@@ -987,7 +1008,7 @@ RK_DEV void sort_and_write(const KParams& p, const RowDesc& d, uint64_t* sbuf, u
 // ---------------------------------------------------------------------------
 // At most 80 VGPRs (6 waves per SIMD): three 512-thread workgroups per CU.
 template <int DT, bool LARGEST>
-__global__ __launch_bounds__(SEL_NT) __attribute__((amdgpu_waves_per_eu(6)))
+__global__ __launch_bounds__(SEL_NT) __attribute__((amdgpu_waves_per_eu(5)))
 void k_select_finalize(KParams p) {
   if (p.round && p.hdr->fallback_rows == 0) return;
   using U = typename Traits<DT>::U;
@@ -998,14 +1019,17 @@ void k_select_finalize(KParams p) {
   __shared__ uint32_t outv[4];
   __shared__ uint32_t slot_cnt;
   __shared__ uint32_t next_sh;
+  const SortLds sl{hist, scratch64, scratch};
   const int mode = order_mode(p.flags);
   for (;;) {
     if (threadIdx.x == 0) next_sh = atomicAdd(&p.hdr->q_select[p.round], 1u);
     __syncthreads();
-    const uint32_t qi = next_sh;
+    // wave-uniform (readfirstlane tells the compiler): the row's metadata
+    // then comes in scalar loads and stays in scalar registers
+    const uint32_t qi = __builtin_amdgcn_readfirstlane(next_sh);
     __syncthreads();
     if (qi >= (uint32_t)p.R) break;
-    const int r = (int)p.order[qi];
+    const int r = __builtin_amdgcn_readfirstlane((int)p.order[qi]);
     const uint32_t ph = p.st[r].phase;
     if (ph != PH_COPYALL && ph != PH_SMALL && ph != PH_FILTER) continue;
     uint32_t next_phase = PH_DONE;
@@ -1032,7 +1056,7 @@ void k_select_finalize(KParams p) {
         return order_key<DT>(mode, sel_key<DT, LARGEST>((uint32_t)x), i);
       });
       __syncthreads();
-      sort_and_write<DT, LARGEST>(p, d, sbuf, m, mode);
+      sort_and_write<DT, LARGEST>(p, d, sbuf, m, mode, sl);
     } else {
       const uint64_t* keys;
       uint32_t n;
@@ -1159,7 +1183,7 @@ void k_select_finalize(KParams p) {
           for (uint32_t i = threadIdx.x; i < k; i += SEL_NT) mk[i] = sbuf[i];
         next_phase = PH_MERGE;
       } else {
-        sort_and_write<DT, LARGEST>(p, d, sbuf, k, mode, r);
+        sort_and_write<DT, LARGEST>(p, d, sbuf, k, mode, sl, r);
       }
       RK_STAMP(p, r, 12);
     }
@@ -1803,6 +1827,10 @@ __global__ __launch_bounds__(SEL_NT) void k_finalize_ordered(KParams p) {
   if (p.hdr->ordered_rows == 0) return;
   using U = typename Traits<DT>::U;
   extern __shared__ uint64_t sbuf[];
+  __shared__ uint32_t bins[HIST_MAX_BINS];
+  __shared__ uint64_t s64[2 * SEL_NT / 64];
+  __shared__ uint32_t s32[SEL_NT / 64];
+  const SortLds sl{bins, s64, s32};
   const int mode = order_mode(p.flags);
   for (int r = blockIdx.x; r < p.R; r += gridDim.x) {
     if (p.st[r].phase != PH_ORDERED) continue;
@@ -1825,7 +1853,7 @@ __global__ __launch_bounds__(SEL_NT) void k_finalize_ordered(KParams p) {
         sbuf[i] = order_key<DT>(mode, sel_key<DT, LARGEST>((uint32_t)ov[i]),
                                 (uint32_t)get_idx(p, d, i));
       __syncthreads();
-      sort_and_write<DT, LARGEST>(p, d, sbuf, m, mode);
+      sort_and_write<DT, LARGEST>(p, d, sbuf, m, mode, sl);
     }
     __syncthreads();
     if (threadIdx.x == 0) p.st[r].phase = PH_DONE;
