@@ -611,72 +611,72 @@ RK_DEV void block_bitonic_desc(uint64_t* a, uint32_t P2) {
   else bitonic_sort_lds<NT>(a, P2);
 }
 
-// Descending sort of m distinct 64-bit keys a[0, m), m <= NT * E, E keys
-// per thread in registers: one most-significant-digit pass and ranks within
-// buckets.  The 11 bits below the common prefix of the largest and the
-// smallest key pick one of 2048 buckets (bucket order is key order); the
-// keys are grouped by bucket, and a key's place in its bucket is the number
-// of the bucket's keys above it.  Work O(m + sum of squared bucket sizes),
-// seven barriers, where a bitonic network does O(m log^2 m) work in
-// log^2 m / 2 dependent stages.  Returns false, keys untouched, when a
-// bucket holds more than 64 keys (skewed or tied keys): the caller sorts
-// them another way.  bins: 2048 words of LDS; scratch64: 2 NT / 64 words.
+// Descending sort of m distinct 64-bit keys a[0, m) (LDS), m <= NT * E:
+// one most-significant-digit pass into 2048 buckets, then each key's place
+// is its bucket's start plus the number of the bucket's keys above it.
+//
+// A key's bucket is the top 11 of its varying bits: the bits of its high
+// 32-bit half that differ somewhere in the set, followed by those of its low
+// half.  Both halves compare as unsigned integers, high half first, so
+// dropping the bits every key shares within each half keeps the order.  A
+// composite key (value key << 32 | ~position) has constant bits at the top
+// of the position half (rows are shorter than 2^32); the common prefix of
+// the whole 64-bit key would stop at the value bits and put every copy of a
+// repeated value in one bucket, where the varying bits spread the copies by
+// position.
+//
+// Ranks are counted in grouped order, so the 64 lanes of a wave mostly
+// share one bucket, its loop and its (broadcast) LDS reads: about
+// m / 64 + sum over buckets of size^2 / 64 wave steps, where a bitonic
+// network takes log^2 m / 2 dependent stages over all m keys.  Returns
+// false, keys untouched, when the sum of squared bucket sizes exceeds
+// BUCKET_RANK_BUDGET * m (ties in the top varying bits): the caller sorts
+// another way.  bins: 2048 words of LDS; scratch64: NT / 64 words;
+// scratch: NT / 64 words.
+constexpr uint32_t BUCKET_RANK_BUDGET = 128;
+
 template <int NT, int E>
 RK_DEV bool bucket_sort_desc(uint64_t* a, uint32_t m, uint32_t* bins, uint64_t* scratch64,
                              uint32_t* scratch) {
-  constexpr int NB = 2048, PER = NB / NT, NW = NT / 64;
+  constexpr int NB = 2048, PER = NB / NT;
   static_assert(NB % NT == 0, "buckets per thread");
   const uint32_t t = threadIdx.x;
   // Branch-free per key (selects, no divergent writes of the register
   // arrays, which would make the compiler move them to scratch memory).
+  const uint64_t x0 = a[0];
   uint64_t v[E];
-  uint64_t mx = 0, mn = ~0ull;
+  uint64_t diff = 0;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const uint32_t i = t + (uint32_t)e * NT;
-    const bool in = i < m;
-    const uint64_t x = a[in ? i : 0u];
-    v[e] = in ? x : 0ull;
-    mx = (in && x > mx) ? x : mx;
-    mn = (in && x < mn) ? x : mn;
-  }
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) {
-    const uint64_t x = __shfl_xor(mx, d, 64), y = __shfl_xor(mn, d, 64);
-    mx = x > mx ? x : mx;
-    mn = y < mn ? y : mn;
-  }
-  if ((t & 63) == 0) {
-    scratch64[t >> 6] = mx;
-    scratch64[NW + (t >> 6)] = mn;
+    v[e] = a[i < m ? i : 0u];  // keys past m read a[0]: no difference
+    diff |= v[e] ^ x0;
   }
   for (uint32_t i = t; i < (uint32_t)NB; i += NT) bins[i] = 0;
-  __syncthreads();
-  mx = 0;
-  mn = ~0ull;
-#pragma unroll
-  for (int w = 0; w < NW; ++w) {
-    mx = scratch64[w] > mx ? scratch64[w] : mx;
-    mn = scratch64[NW + w] < mn ? scratch64[NW + w] : mn;
-  }
-  const int L = mx == mn ? 63 : __builtin_clzll(mx ^ mn);  // common high bits
-  const int sh = 64 - L - 11 > 0 ? 64 - L - 11 : 0;
-  auto bucket = [&](uint64_t x) { return (uint32_t)(x >> sh) & (NB - 1); };
+  diff = block_or64<NT>(diff, scratch64);  // its barriers also order the clearing
+  const uint32_t dh = (uint32_t)(diff >> 32), dl = (uint32_t)diff;
+  const int kb = dh ? 32 - __builtin_clz(dh) : 0;  // varying bits of the high half
+  const int pb = dl ? 32 - __builtin_clz(dl) : 0;  // and of the low half
+  const int sh = kb + pb > 11 ? kb + pb - 11 : 0;
+  const uint64_t kmask = (1ull << kb) - 1ull, pmask = (1ull << pb) - 1ull;
+  auto bucket = [&](uint64_t x) {
+    return (uint32_t)(((((x >> 32) & kmask) << pb) | (x & pmask)) >> sh);
+  };
 #pragma unroll
   for (int e = 0; e < E; ++e)
     if (t + (uint32_t)e * NT < m) atomicAdd(&bins[bucket(v[e])], 1u);
   __syncthreads();
   // Bucket starts in descending bucket order: thread t scans buckets
-  // NB-1-t*PER down to NB-PER-t*PER; the largest bucket decides the method.
-  uint32_t c[PER], tot = 0, big = 0;
+  // NB-1-t*PER down to NB-PER-t*PER; the squared sizes decide the method.
+  uint32_t c[PER], tot = 0, sq = 0;
 #pragma unroll
   for (int j = 0; j < PER; ++j) {
     c[j] = bins[NB - 1 - (t * PER + j)];
     tot += c[j];
-    big = c[j] > big ? c[j] : big;
+    sq += c[j] * c[j];  // <= m^2 < 2^32
   }
-  big = block_sum<NT>(big > 64 ? 1u : 0u, scratch);
-  if (big != 0) return false;  // uniform
+  sq = block_sum<NT>(sq, scratch);
+  if (sq > BUCKET_RANK_BUDGET * m) return false;  // uniform
   uint32_t all;
   uint32_t run = block_excl_scan<NT>(tot, scratch, all);
 #pragma unroll
@@ -690,11 +690,14 @@ RK_DEV bool bucket_sort_desc(uint64_t* a, uint32_t m, uint32_t* bins, uint64_t* 
   for (int e = 0; e < E; ++e)
     if (t + (uint32_t)e * NT < m) a[atomicAdd(&bins[bucket(v[e])], 1u)] = v[e];
   __syncthreads();
-  // bins[b] is now the end of bucket b, which bucket b - 1 starts at
+  // bins[b] is now the end of bucket b, which bucket b - 1 starts at.  Each
+  // thread ranks the keys at grouped positions t, t + NT, ...
   uint32_t o[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const bool in = t + (uint32_t)e * NT < m;
+    const uint32_t i = t + (uint32_t)e * NT;
+    const bool in = i < m;
+    v[e] = a[in ? i : 0u];
     const uint32_t bk = bucket(v[e]);
     const uint32_t st = bk == NB - 1 ? 0u : bins[bk + 1];
     const uint32_t en = in ? bins[bk] : st;  // no keys to rank against

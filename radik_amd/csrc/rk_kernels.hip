@@ -885,11 +885,13 @@ RK_DEV void filter_segment(const KParams& p, FilterLds& lds, const RowState& st,
 }
 
 // The sampled round's streaming pass over every PH_FILTER row.  At most 72
-// VGPRs (7 waves per SIMD) for 2 loads in flight and 80 (6 waves) for 4, so
-// that the rarely taken candidate path costs no occupancy and no spills.
+// VGPRs (7 waves per SIMD) for 2 loads in flight and 96 (5 waves) for 4:
+// the grids the autotuner offers keep at most 4 workgroups (4 waves per
+// SIMD) per CU, and at 80 registers the 4-load form spills to scratch
+// memory (gfx950: 68-84 bytes per lane).
 template <int DT, bool LARGEST, int U>
 __global__ __launch_bounds__(STREAM_NT)
-__attribute__((amdgpu_waves_per_eu(U >= 8 ? 1 : (U == 4 ? 6 : 7))))
+__attribute__((amdgpu_waves_per_eu(U >= 8 ? 1 : (U == 4 ? 5 : 7))))
 void k_filter(KParams p) {
   __shared__ FilterLds lds;
   for_each_chunk_queued(p, &p.hdr->q_chunk[0], [&](int r, uint32_t ta, uint32_t tb) {
@@ -955,7 +957,7 @@ RK_DEV uint32_t wave_slot(uint32_t* cnt, bool take) {
 // scratch.
 struct SortLds {
   uint32_t* bins;
-  uint64_t* s64;  // >= 2 * SEL_NT / 64 words
+  uint64_t* s64;  // >= SEL_NT / 64 words
   uint32_t* s32;  // >= SEL_NT / 64 words
 };
 
@@ -964,9 +966,10 @@ struct SortLds {
 // a power of two).
 RK_DEV void sort_desc(uint64_t* sbuf, uint32_t m, const SortLds& sl) {
   __syncthreads();  // sbuf complete
-  if (m > 64 && m <= 4u * SEL_NT) {
-    const bool ok = m <= 2u * SEL_NT ? bucket_sort_desc<SEL_NT, 2>(sbuf, m, sl.bins, sl.s64, sl.s32)
-                                     : bucket_sort_desc<SEL_NT, 4>(sbuf, m, sl.bins, sl.s64, sl.s32);
+  if (m > 64 && m <= 8u * SEL_NT) {
+    const bool ok = m <= 2u * SEL_NT   ? bucket_sort_desc<SEL_NT, 2>(sbuf, m, sl.bins, sl.s64, sl.s32)
+                    : m <= 4u * SEL_NT ? bucket_sort_desc<SEL_NT, 4>(sbuf, m, sl.bins, sl.s64, sl.s32)
+                                       : bucket_sort_desc<SEL_NT, 8>(sbuf, m, sl.bins, sl.s64, sl.s32);
     if (ok) return;
   }
   const uint32_t P2 = next_pow2(m);
@@ -1006,9 +1009,12 @@ RK_DEV void sort_and_write(const KParams& p, const RowDesc& d, uint64_t* sbuf, u
 // largest composite keys, sort and write.  A sampled row that fails the
 // verification goes to the exact passes.
 // ---------------------------------------------------------------------------
-// At most 80 VGPRs (6 waves per SIMD): three 512-thread workgroups per CU.
+// Two 512-thread workgroups per CU (4 waves per SIMD, up to 128 VGPRs): the
+// select's loops hold 8 loads in flight and the 4096-key bucket sort 8 keys
+// per thread in registers, which a 96- or 80-register budget spills to
+// scratch memory (gfx950, 96: 72-156 bytes per lane).
 template <int DT, bool LARGEST>
-__global__ __launch_bounds__(SEL_NT) __attribute__((amdgpu_waves_per_eu(5)))
+__global__ __launch_bounds__(SEL_NT) __attribute__((amdgpu_waves_per_eu(4)))
 void k_select_finalize(KParams p) {
   if (p.round && p.hdr->fallback_rows == 0) return;
   using U = typename Traits<DT>::U;
@@ -1236,6 +1242,9 @@ constexpr int MS_E = MERGE_CHUNK / MS_NT;   // keys per thread (registers)
 template <int DT>
 __global__ __launch_bounds__(MS_NT) void k_merge_sort(KParams p) {
   __shared__ uint64_t xs[MERGE_CHUNK];
+  __shared__ uint32_t bins[2048];
+  __shared__ uint64_t s64[MS_NT / 64];
+  __shared__ uint32_t s32[MS_NT / 64];
   const int cpr = merge_chunks(p);
   const int64_t items = (int64_t)p.R * cpr;
   for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
@@ -1248,20 +1257,29 @@ __global__ __launch_bounds__(MS_NT) void k_merge_sort(KParams p) {
     uint64_t* keys = p.cand + p.cand_base[r] + c0;
     [[maybe_unused]] const int sb = 16 + 4 * (int)(it % cpr);  // phase-timing slots
     RK_STAMP(p, r, sb);
-    uint64_t v[MS_E];
-#pragma unroll
-    for (int e = 0; e < MS_E; ++e) {
-      const uint32_t i = threadIdx.x * MS_E + e;
-      v[e] = i < n ? ldg(keys + i) : 0ull;  // zero pads: sorts last
-    }
+    load_to_lds<MS_NT, MS_E>(xs, keys, n, [](uint64_t x, uint32_t) { return x; });
+    __syncthreads();
     RK_STAMP(p, r, sb + 1);
-    bitonic_sort_regs<MS_NT, MS_E>(v, xs);
-    RK_STAMP(p, r, sb + 2);
+    // Buckets and ranks when the keys spread over their top varying bits,
+    // else the bitonic network in registers (zeros pad it: they sort last).
+    if (n > 64 && bucket_sort_desc<MS_NT, MS_E>(xs, n, bins, s64, s32)) {
+      for (uint32_t i = threadIdx.x; i < n; i += MS_NT) keys[i] = xs[i];
+    } else {
+      uint64_t v[MS_E];
 #pragma unroll
-    for (int e = 0; e < MS_E; ++e) {
-      const uint32_t i = threadIdx.x * MS_E + e;
-      if (i < n) keys[i] = v[e];
+      for (int e = 0; e < MS_E; ++e) {
+        const uint32_t i = threadIdx.x * MS_E + e;
+        v[e] = i < n ? xs[i] : 0ull;
+      }
+      __syncthreads();  // xs is the network's exchange buffer from here on
+      bitonic_sort_regs<MS_NT, MS_E>(v, xs);
+#pragma unroll
+      for (int e = 0; e < MS_E; ++e) {
+        const uint32_t i = threadIdx.x * MS_E + e;
+        if (i < n) keys[i] = v[e];
+      }
     }
+    RK_STAMP(p, r, sb + 2);
     __syncthreads();
     RK_STAMP(p, r, sb + 3);
   }

@@ -422,3 +422,29 @@ def test_large_k_chunked_sort(dt, flags):
     idx_in = rng.integers(-2**31, 2**31 - 1, int(off[-1])).astype(np.int32)
     _check(dt, bits, off, ks, flags)
     _check(dt, bits, off, ks, flags, idx_in=idx_in)
+
+
+@pytest.mark.parametrize("dt", ["f32", "bf16", "f16"])
+@pytest.mark.parametrize("flags", FLAG_SETS)
+def test_bucket_rank_sort_paths(dt, flags):
+    """The final sort of a selection by buckets and ranks (2, 4 and 8 keys
+    per thread; the 4096-key merge chunks) on selections whose values
+    repeat hundreds of times (the buckets must spread copies by position),
+    and a selection whose top varying bits are shared by nearly every key
+    (a far outlier: the sort falls back to the bitonic network)."""
+    rng = np.random.default_rng(901 + DT_CODE[dt] * 11 + flags)
+    lens = np.array([400_000, 300_000, 500_000, 1 << 20, 100_000, 80_000, 250_000], np.int64)
+    ks = np.array([3000, 4096, 2500, 16384, 1500, 700, 3500], np.int32)
+    off = np.zeros(len(lens) + 1, np.int64)
+    off[1:] = np.cumsum(lens)
+    parts = []
+    for r, n in enumerate(lens):
+        x = rng.standard_normal(int(n)).astype(np.float32)
+        if r in (0, 1, 3, 4, 6):
+            x = np.round(x * 4.0).astype(np.float32) / np.float32(4.0)  # values repeat
+        if r in (1, 6):
+            x[rng.integers(0, n)] = np.float32(3.0e4 if dt == "f16" else 1.0e30)
+            x[rng.integers(0, n)] = np.float32(-3.0e4 if dt == "f16" else -1.0e30)
+        parts.append(_bits_of(dt, x))
+    bits = np.concatenate(parts)
+    _check(dt, bits, off, ks, flags)
