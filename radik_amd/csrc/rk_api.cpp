@@ -198,8 +198,10 @@ int launch(int dtype, const void* vals, const int64_t* row_off, const void* cons
   p.cand_capacity = L.cand_cap;
   // LDS of the select kernels sized to this call: the k selected keys (and
   // rows up to that length) live in LDS; larger candidate sets are selected
-  // from global memory.  Smaller buffers let more workgroups share a CU.
-  int lk = 4096;
+  // from global memory.  At least 8192 keys (64 KB): the select kernel's
+  // registers hold it at two workgroups per CU, which leave 80 KB of LDS
+  // each, so the smaller buffer would buy no occupancy.
+  int lk = 8192;
   while (lk < max_k && lk < di.lds_keys) lk <<= 1;
   p.lds_keys = std::min(lk, di.lds_keys);
   p.small_max = p.lds_keys;

@@ -230,26 +230,95 @@ RK_DEV uint32_t mbcnt(uint64_t mask) {
                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-RK_DEV uint32_t wave_incl_scan(uint32_t x) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    uint32_t y = __shfl_up(x, d, 64);
-    if (lane >= d) x += y;
+// Wave64 scans and reductions on DPP (data-parallel primitive) moves: four
+// row_shr steps (distances 1, 2, 4, 8) scan or reduce each row of 16 lanes
+// in VALU operations, and readlane of the rows' last lanes (15, 31, 47, 63)
+// joins the four rows in scalar registers.  (Lane shuffles would go through
+// the LDS crossbar, ds_bpermute, with its latency on every step.)  Called
+// with the whole wave active.
+//
+// row_shr:n (DPP control 0x110 + n, gfx9 family): lane i of a row reads lane
+// i - n of the same row; a lane whose source lies before the row keeps
+// `id`, the operation's identity.
+template <int N>
+RK_DEV uint32_t row_shr(uint32_t v, uint32_t id) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, 0x110 + N, 0xf, 0xf, false);
+}
+template <int N>
+RK_DEV uint64_t row_shr64(uint64_t v, uint64_t id) {
+  return ((uint64_t)row_shr<N>((uint32_t)(v >> 32), (uint32_t)(id >> 32)) << 32) |
+         row_shr<N>((uint32_t)v, (uint32_t)id);
+}
+RK_DEV uint32_t readlane32(uint32_t v, int l) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+RK_DEV uint64_t readlane64(uint64_t v, int l) {
+  return ((uint64_t)readlane32((uint32_t)(v >> 32), l) << 32) | readlane32((uint32_t)v, l);
+}
+
+// Inclusive scan (op) of each row of 16 lanes.
+template <typename T, typename Op>
+RK_DEV T row_scan(T x, T id, Op op) {
+  if constexpr (sizeof(T) == 8) {
+    x = op(x, row_shr64<1>(x, id));
+    x = op(x, row_shr64<2>(x, id));
+    x = op(x, row_shr64<4>(x, id));
+    x = op(x, row_shr64<8>(x, id));
+  } else {
+    x = op(x, row_shr<1>(x, id));
+    x = op(x, row_shr<2>(x, id));
+    x = op(x, row_shr<4>(x, id));
+    x = op(x, row_shr<8>(x, id));
   }
   return x;
 }
 
+// Reduction (op) over the wave, the same value in every lane.
+template <typename T, typename Op>
+RK_DEV T wave_reduce(T x, T id, Op op) {
+  x = row_scan(x, id, op);
+  if constexpr (sizeof(T) == 8)
+    return op(op(readlane64(x, 15), readlane64(x, 31)), op(readlane64(x, 47), readlane64(x, 63)));
+  else
+    return op(op(readlane32(x, 15), readlane32(x, 31)), op(readlane32(x, 47), readlane32(x, 63)));
+}
+
+// Inclusive sum over the lanes up to this one.
+template <typename T>
+RK_DEV T wave_incl_scan_t(T x) {
+  auto add = [](T a, T b) { return a + b; };
+  x = row_scan(x, T(0), add);
+  T r0, r1, r2;
+  if constexpr (sizeof(T) == 8) {
+    r0 = readlane64(x, 15);
+    r1 = readlane64(x, 31);
+    r2 = readlane64(x, 47);
+  } else {
+    r0 = readlane32(x, 15);
+    r1 = readlane32(x, 31);
+    r2 = readlane32(x, 47);
+  }
+  const int row = lane_id() >> 4;
+  return x + (row >= 1 ? r0 : T(0)) + (row >= 2 ? r1 : T(0)) + (row >= 3 ? r2 : T(0));
+}
+
+RK_DEV uint32_t wave_incl_scan(uint32_t x) { return wave_incl_scan_t<uint32_t>(x); }
+RK_DEV uint64_t wave_incl_scan64(uint64_t x) { return wave_incl_scan_t<uint64_t>(x); }
+
 RK_DEV uint32_t wave_sum(uint32_t x) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
-  return x;
+  return wave_reduce<uint32_t>(x, 0u, [](uint32_t a, uint32_t b) { return a + b; });
 }
 
 RK_DEV uint64_t wave_or64(uint64_t x) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) x |= __shfl_xor(x, d, 64);
-  return x;
+  return wave_reduce<uint64_t>(x, 0ull, [](uint64_t a, uint64_t b) { return a | b; });
+}
+
+RK_DEV uint64_t wave_min64(uint64_t x) {
+  return wave_reduce<uint64_t>(x, ~0ull, [](uint64_t a, uint64_t b) { return a < b ? a : b; });
+}
+
+RK_DEV uint64_t wave_max64(uint64_t x) {
+  return wave_reduce<uint64_t>(x, 0ull, [](uint64_t a, uint64_t b) { return a > b ? a : b; });
 }
 
 // Exclusive scan over the workgroup in thread order.  scratch: >= NT/64 words.
@@ -296,6 +365,75 @@ RK_DEV uint64_t block_or64(uint64_t x, uint64_t* scratch) {
   for (int w = 0; w < NW; ++w) r |= scratch[w];
   __syncthreads();
   return r;
+}
+
+// Workgroup minimum and maximum of 64-bit values.  scratch: >= 2 NT/64 words.
+template <int NT>
+RK_DEV void block_minmax64(uint64_t& lo, uint64_t& hi, uint64_t* scratch) {
+  constexpr int NW = NT / 64;
+  lo = wave_min64(lo);
+  hi = wave_max64(hi);
+  if (lane_id() == 0) {
+    scratch[threadIdx.x >> 6] = lo;
+    scratch[NW + (threadIdx.x >> 6)] = hi;
+  }
+  __syncthreads();
+  lo = ~0ull;
+  hi = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    lo = scratch[w] < lo ? scratch[w] : lo;
+    hi = scratch[NW + w] > hi ? scratch[NW + w] : hi;
+  }
+  __syncthreads();
+}
+
+// Exclusive 64-bit scan over the workgroup in thread order.  scratch: >=
+// NT/64 words.
+template <int NT>
+RK_DEV uint64_t block_excl_scan64(uint64_t x, uint64_t* scratch, uint64_t& total) {
+  constexpr int NW = NT / 64;
+  const int wid = threadIdx.x >> 6;
+  const uint64_t inc = wave_incl_scan64(x);
+  if (lane_id() == 63) scratch[wid] = inc;
+  __syncthreads();
+  uint64_t pre = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    const uint64_t v = scratch[w];
+    pre += (w < wid) ? v : 0ull;
+    tot += v;
+  }
+  __syncthreads();
+  total = tot;
+  return pre + inc - x;
+}
+
+// Workgroup OR, minimum and maximum of 64-bit values in one exchange.
+// scratch: >= 3 NT/64 words.
+template <int NT>
+RK_DEV void block_or_min_max64(uint64_t& o, uint64_t& lo, uint64_t& hi, uint64_t* scratch) {
+  constexpr int NW = NT / 64;
+  o = wave_or64(o);
+  lo = wave_min64(lo);
+  hi = wave_max64(hi);
+  const int wid = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+    scratch[wid] = o;
+    scratch[NW + wid] = lo;
+    scratch[2 * NW + wid] = hi;
+  }
+  __syncthreads();
+  o = 0;
+  lo = ~0ull;
+  hi = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    o |= scratch[w];
+    lo = scratch[NW + w] < lo ? scratch[NW + w] : lo;
+    hi = scratch[2 * NW + w] > hi ? scratch[2 * NW + w] : hi;
+  }
+  __syncthreads();
 }
 
 // dst[i] = f(src[i], i) for i < n, over the workgroup, with UN global loads
@@ -611,63 +749,145 @@ RK_DEV void block_bitonic_desc(uint64_t* a, uint32_t P2) {
   else bitonic_sort_lds<NT>(a, P2);
 }
 
-// Descending sort of m distinct 64-bit keys a[0, m) (LDS), m <= NT * E:
-// one most-significant-digit pass into 2048 buckets, then each key's place
-// is its bucket's start plus the number of the bucket's keys above it.
+// Descending sort of m distinct 64-bit keys a[0, m) (LDS), using tmp[0, m)
+// (LDS, disjoint from a): one most-significant-digit pass groups the keys
+// by bucket into tmp, then each key goes back to a at its bucket's start
+// plus the number of the bucket's keys above it.  Registers hold no keys
+// across the passes, so the sort costs the calling kernel no occupancy.
 //
-// A key's bucket is the top 11 of its varying bits: the bits of its high
-// 32-bit half that differ somewhere in the set, followed by those of its low
-// half.  Both halves compare as unsigned integers, high half first, so
-// dropping the bits every key shares within each half keeps the order.  A
-// composite key (value key << 32 | ~position) has constant bits at the top
-// of the position half (rows are shorter than 2^32); the common prefix of
-// the whole 64-bit key would stop at the value bits and put every copy of a
-// repeated value in one bucket, where the varying bits spread the copies by
-// position.
+// Buckets follow the distribution of the keys' squeezed images.  The
+// squeezed image of a key is its high 32-bit half without the bits that
+// every key shares there, followed by its low half without its shared bits.
+// Both halves compare as unsigned integers, high half first, so squeezing
+// keeps the order; it removes the constant top bits of the position half of
+// a composite key (value key << 32 | ~position; rows are shorter than 2^32),
+// so the copies of a repeated value spread over buckets by position when
+// the values are close in key space.  The images' range is cut linearly
+// into 256 coarse bins; each coarse bin gets fine buckets in proportion to
+// its keys, and a key's fine bucket in its coarse bin is linear in its
+// offset inside the bin.  So the buckets adapt to clustered keys (the
+// float keys of a row sit in a few exponents, in two ranges far apart
+// when both signs occur) where a single linear or top-bits cut would leave
+// most buckets empty.
 //
 // Ranks are counted in grouped order, so the 64 lanes of a wave mostly
 // share one bucket, its loop and its (broadcast) LDS reads: about
 // m / 64 + sum over buckets of size^2 / 64 wave steps, where a bitonic
-// network takes log^2 m / 2 dependent stages over all m keys.  Returns
+// network takes log^2 m / 2 dependent stages over all m keys.  Every pass
+// issues a thread's LDS loads four at a time, before any store.  Returns
 // false, keys untouched, when the sum of squared bucket sizes exceeds
-// BUCKET_RANK_BUDGET * m (ties in the top varying bits): the caller sorts
-// another way.  bins: 2048 words of LDS; scratch64: NT / 64 words;
-// scratch: NT / 64 words.
+// BUCKET_RANK_BUDGET * m (a few values far apart in key space, each
+// repeated many times; or a far outlier): the caller sorts another way.
+// m <= BUCKET_SORT_MAX.  bins: 2048 words of LDS; ctab: 256 words of LDS;
+// scratch64: 3 NT / 64 words; scratch: NT / 64 words.
 constexpr uint32_t BUCKET_RANK_BUDGET = 128;
+constexpr uint32_t BUCKET_SORT_MAX = 4096;
 
-template <int NT, int E>
-RK_DEV bool bucket_sort_desc(uint64_t* a, uint32_t m, uint32_t* bins, uint64_t* scratch64,
-                             uint32_t* scratch) {
-  constexpr int NB = 2048, PER = NB / NT;
+template <int NT>
+RK_DEV bool bucket_sort_desc(uint64_t* a, uint64_t* tmp, uint32_t m, uint32_t* bins,
+                             uint32_t* ctab, uint64_t* scratch64, uint32_t* scratch) {
+  constexpr int NB = 2048, NC = 256, PER = NB / NT, UN = 4;
   static_assert(NB % NT == 0, "buckets per thread");
   const uint32_t t = threadIdx.x;
-  // Branch-free per key (selects, no divergent writes of the register
-  // arrays, which would make the compiler move them to scratch memory).
-  const uint64_t x0 = a[0];
-  uint64_t v[E];
-  uint64_t diff = 0;
+  // Loads UN keys per thread (keys past m read a[0], within the keys'
+  // range; callers mask them out by index).
+  auto load4 = [&](const uint64_t* src, uint32_t i0, uint64_t (&x)[UN]) {
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const uint32_t i = t + (uint32_t)e * NT;
-    v[e] = a[i < m ? i : 0u];  // keys past m read a[0]: no difference
-    diff |= v[e] ^ x0;
+    for (int u = 0; u < UN; ++u) {
+      const uint32_t i = i0 + (uint32_t)u * NT;
+      x[u] = src[i < m ? i : 0u];
+    }
+  };
+  // One pass: the bits that vary, and the smallest and largest key.  The
+  // squeezed image is monotone in the key, so the images' range is the
+  // images of these two.
+  const uint64_t x0 = a[0];
+  uint64_t diff = 0, kmin = ~0ull, kmax = 0;
+  for (uint32_t i0 = t; i0 < m; i0 += NT * UN) {
+    uint64_t x[UN];
+    load4(a, i0, x);
+#pragma unroll
+    for (int u = 0; u < UN; ++u) {
+      diff |= x[u] ^ x0;
+      kmin = x[u] < kmin ? x[u] : kmin;
+      kmax = x[u] > kmax ? x[u] : kmax;
+    }
   }
   for (uint32_t i = t; i < (uint32_t)NB; i += NT) bins[i] = 0;
-  diff = block_or64<NT>(diff, scratch64);  // its barriers also order the clearing
+  for (uint32_t i = t; i < (uint32_t)NC; i += NT) ctab[i] = 0;
+  // its barriers also order the clearing
+  block_or_min_max64<NT>(diff, kmin, kmax, scratch64);
   const uint32_t dh = (uint32_t)(diff >> 32), dl = (uint32_t)diff;
   const int kb = dh ? 32 - __builtin_clz(dh) : 0;  // varying bits of the high half
   const int pb = dl ? 32 - __builtin_clz(dl) : 0;  // and of the low half
-  const int sh = kb + pb > 11 ? kb + pb - 11 : 0;
   const uint64_t kmask = (1ull << kb) - 1ull, pmask = (1ull << pb) - 1ull;
-  auto bucket = [&](uint64_t x) {
-    return (uint32_t)(((((x >> 32) & kmask) << pb) | (x & pmask)) >> sh);
-  };
+  auto squeeze = [&](uint64_t x) { return (((x >> 32) & kmask) << pb) | (x & pmask); };
+  const uint64_t lo = squeeze(kmin), hi = squeeze(kmax);
+  // Position of an image in [lo, hi] as the 64-bit fixed-point number
+  // (d >> s) * mul = coarse bin (high 32 bits) . fraction (low 32 bits),
+  // mul = floor(NC 2^32 / r), r = (span >> s) + 1: the offset is cut to its
+  // top 32 bits (s), so r <= 2^32 and the product stays below NC 2^32;
+  // monotone in the image.  A span below NB takes the offset as the bucket.
+  const uint64_t span = hi - lo;
+  const bool direct = span < (uint64_t)NB;
+  const int bl = span ? 64 - __builtin_clzll(span) : 0;
+  const int s = bl > 32 ? bl - 32 : 0;
+  // mul = floor(NC 2^32 / r): a double quotient, corrected to the floor by
+  // exact integer products (mul * r <= NC 2^32 + r < 2^64)
+  uint64_t mul = 0;
+  if (!direct) {
+    const uint64_t r = (span >> s) + 1ull, num = (uint64_t)NC << 32;
+    mul = (uint64_t)((double)num / (double)r);
+    while (mul * r > num) --mul;
+    while ((mul + 1) * r <= num) ++mul;
+  }
+  auto fixed = [&](uint64_t x) { return ((squeeze(x) - lo) >> s) * mul; };
+  if (!direct) {
+    for (uint32_t i0 = t; i0 < m; i0 += NT * UN) {
+      uint64_t x[UN];
+      load4(a, i0, x);
 #pragma unroll
-  for (int e = 0; e < E; ++e)
-    if (t + (uint32_t)e * NT < m) atomicAdd(&bins[bucket(v[e])], 1u);
+      for (int u = 0; u < UN; ++u)
+        if (i0 + (uint32_t)u * NT < m) atomicAdd(&ctab[(uint32_t)(fixed(x[u]) >> 32)], 1u);
+    }
+    __syncthreads();
+    // fine buckets of coarse bin c: 1 + floor(n_c (NB - NC) / m) when it has
+    // keys (at most NB in all), packed as first bucket | count << 12
+    if (t < 64) {
+      uint32_t f[NC / 64], tot = 0;
+#pragma unroll
+      for (int j = 0; j < NC / 64; ++j) {
+        const uint32_t n = ctab[t * (NC / 64) + j];
+        f[j] = n ? 1u + n * (uint32_t)(NB - NC) / m : 0u;
+        tot += f[j];
+      }
+      uint32_t run = wave_incl_scan(tot) - tot;
+#pragma unroll
+      for (int j = 0; j < NC / 64; ++j) {
+        ctab[t * (NC / 64) + j] = run | (f[j] << 12);
+        run += f[j];
+      }
+    }
+    __syncthreads();
+  }
+  auto bucket = [&](uint64_t x) -> uint32_t {
+    if (direct) return (uint32_t)(squeeze(x) - lo);
+    const uint64_t fx = fixed(x);
+    const uint32_t e = ctab[(uint32_t)(fx >> 32)];
+    return (e & 0xfffu) + (uint32_t)(((uint64_t)(uint32_t)fx * (e >> 12)) >> 32);
+  };
+  for (uint32_t i0 = t; i0 < m; i0 += NT * UN) {
+    uint64_t x[UN];
+    load4(a, i0, x);
+#pragma unroll
+    for (int u = 0; u < UN; ++u)
+      if (i0 + (uint32_t)u * NT < m) atomicAdd(&bins[bucket(x[u])], 1u);
+  }
   __syncthreads();
   // Bucket starts in descending bucket order: thread t scans buckets
   // NB-1-t*PER down to NB-PER-t*PER; the squared sizes decide the method.
+  // One 64-bit scan carries the key counts (low half, at most m) and the
+  // squared sizes (high half, at most m^2), whose total is all it needs.
   uint32_t c[PER], tot = 0, sq = 0;
 #pragma unroll
   for (int j = 0; j < PER; ++j) {
@@ -675,40 +895,48 @@ RK_DEV bool bucket_sort_desc(uint64_t* a, uint32_t m, uint32_t* bins, uint64_t* 
     tot += c[j];
     sq += c[j] * c[j];  // <= m^2 < 2^32
   }
-  sq = block_sum<NT>(sq, scratch);
-  if (sq > BUCKET_RANK_BUDGET * m) return false;  // uniform
-  uint32_t all;
-  uint32_t run = block_excl_scan<NT>(tot, scratch, all);
+  uint64_t all;
+  const uint64_t pre = block_excl_scan64<NT>(((uint64_t)sq << 32) | tot, scratch64, all);
+  if ((uint32_t)(all >> 32) > BUCKET_RANK_BUDGET * m) return false;  // uniform
+  uint32_t run = (uint32_t)pre;
 #pragma unroll
   for (int j = 0; j < PER; ++j) {
     bins[NB - 1 - (t * PER + j)] = run;  // start: the bucket's cursor
     run += c[j];
   }
   __syncthreads();
-  // grouped by bucket (any order inside one)
+  // grouped by bucket into tmp (any order inside one)
+  for (uint32_t i0 = t; i0 < m; i0 += NT * UN) {
+    uint64_t x[UN];
+    load4(a, i0, x);
+    uint32_t bk[UN];
 #pragma unroll
-  for (int e = 0; e < E; ++e)
-    if (t + (uint32_t)e * NT < m) a[atomicAdd(&bins[bucket(v[e])], 1u)] = v[e];
-  __syncthreads();
-  // bins[b] is now the end of bucket b, which bucket b - 1 starts at.  Each
-  // thread ranks the keys at grouped positions t, t + NT, ...
-  uint32_t o[E];
+    for (int u = 0; u < UN; ++u) bk[u] = bucket(x[u]);
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const uint32_t i = t + (uint32_t)e * NT;
-    const bool in = i < m;
-    v[e] = a[in ? i : 0u];
-    const uint32_t bk = bucket(v[e]);
-    const uint32_t st = bk == NB - 1 ? 0u : bins[bk + 1];
-    const uint32_t en = in ? bins[bk] : st;  // no keys to rank against
-    uint32_t r = st;
-    for (uint32_t j = st; j < en; ++j) r += a[j] > v[e] ? 1u : 0u;
-    o[e] = r;
+    for (int u = 0; u < UN; ++u)
+      if (i0 + (uint32_t)u * NT < m) tmp[atomicAdd(&bins[bk[u]], 1u)] = x[u];
   }
   __syncthreads();
+  // bins[b] is now the end of bucket b, which bucket b - 1 starts at.  Each
+  // thread ranks the keys at grouped positions t, t + NT, ... and stores
+  // them at their place in a.
+  for (uint32_t i0 = t; i0 < m; i0 += NT * UN) {
+    uint64_t x[UN];
+    load4(tmp, i0, x);
+    uint32_t st[UN], en[UN];
 #pragma unroll
-  for (int e = 0; e < E; ++e)
-    if (t + (uint32_t)e * NT < m) a[o[e]] = v[e];
+    for (int u = 0; u < UN; ++u) {
+      const uint32_t bk = bucket(x[u]);
+      st[u] = bk == NB - 1 ? 0u : bins[bk + 1];
+      en[u] = i0 + (uint32_t)u * NT < m ? bins[bk] : st[u];  // no keys to rank against
+    }
+#pragma unroll
+    for (int u = 0; u < UN; ++u) {
+      uint32_t r = st[u];
+      for (uint32_t j = st[u]; j < en[u]; ++j) r += tmp[j] > x[u] ? 1u : 0u;
+      if (i0 + (uint32_t)u * NT < m) a[r] = x[u];
+    }
+  }
   __syncthreads();
   return true;
 }

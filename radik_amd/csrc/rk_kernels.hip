@@ -375,13 +375,8 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
     uint32_t tot_t;
     const uint32_t ex_t = block_excl_scan<1024>(t, scratch, tot_t);
     // 64-bit scan of the capacities: wave scan + per-wave totals.
-    uint64_t inc = c;
+    const uint64_t inc = wave_incl_scan64(c);
     const int lane = lane_id(), wid = tid >> 6;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint64_t y = __shfl_up(inc, d, 64);
-      if (lane >= d) inc += y;
-    }
     if (lane == 63) scratch64[wid] = inc;
     __syncthreads();
     uint64_t pre = 0, tot_c = 0;
@@ -885,13 +880,13 @@ RK_DEV void filter_segment(const KParams& p, FilterLds& lds, const RowState& st,
 }
 
 // The sampled round's streaming pass over every PH_FILTER row.  At most 72
-// VGPRs (7 waves per SIMD) for 2 loads in flight and 96 (5 waves) for 4:
+// VGPRs (7 waves per SIMD) for 2 loads in flight and 128 (4 waves) for 4:
 // the grids the autotuner offers keep at most 4 workgroups (4 waves per
-// SIMD) per CU, and at 80 registers the 4-load form spills to scratch
-// memory (gfx950: 68-84 bytes per lane).
+// SIMD) per CU, and at 80 or 96 registers the 4-load form spills to scratch
+// memory (gfx950: 16-84 bytes per lane).
 template <int DT, bool LARGEST, int U>
 __global__ __launch_bounds__(STREAM_NT)
-__attribute__((amdgpu_waves_per_eu(U >= 8 ? 1 : (U == 4 ? 5 : 7))))
+__attribute__((amdgpu_waves_per_eu(U >= 8 ? 1 : (U == 4 ? 4 : 7))))
 void k_filter(KParams p) {
   __shared__ FilterLds lds;
   for_each_chunk_queued(p, &p.hdr->q_chunk[0], [&](int r, uint32_t ta, uint32_t tb) {
@@ -957,35 +952,41 @@ RK_DEV uint32_t wave_slot(uint32_t* cnt, bool take) {
 // scratch.
 struct SortLds {
   uint32_t* bins;
-  uint64_t* s64;  // >= SEL_NT / 64 words
+  uint64_t* s64;  // >= 3 * SEL_NT / 64 words
   uint32_t* s32;  // >= SEL_NT / 64 words
+  uint32_t* ctab; // 256 words (the bucket sort's coarse bins)
+  uint32_t cap;   // keys the sort buffer holds
 };
+
+// Sorts the n (64 < n <= BUCKET_SORT_MAX) distinct keys sbuf[0, n)
+// descending by buckets and ranks, with sbuf[BUCKET_SORT_MAX, 2
+// BUCKET_SORT_MAX) as the grouping buffer (the select's LDS holds at least
+// 8192 keys); false (keys untouched) when they do not spread over the
+// buckets.
+RK_DEV bool bucket_sort_all(uint64_t* sbuf, uint32_t n, const SortLds& sl) {
+  if (n > BUCKET_SORT_MAX || sl.cap < 2 * BUCKET_SORT_MAX) return false;
+  return bucket_sort_desc<SEL_NT>(sbuf, sbuf + BUCKET_SORT_MAX, n, sl.bins, sl.ctab, sl.s64,
+                                  sl.s32);
+}
 
 // Sorts the distinct keys sbuf[0, m) descending: by buckets and ranks when
 // they spread (bucket_sort_desc), else by a bitonic network (zeros pad it to
 // a power of two).
 RK_DEV void sort_desc(uint64_t* sbuf, uint32_t m, const SortLds& sl) {
   __syncthreads();  // sbuf complete
-  if (m > 64 && m <= 8u * SEL_NT) {
-    const bool ok = m <= 2u * SEL_NT   ? bucket_sort_desc<SEL_NT, 2>(sbuf, m, sl.bins, sl.s64, sl.s32)
-                    : m <= 4u * SEL_NT ? bucket_sort_desc<SEL_NT, 4>(sbuf, m, sl.bins, sl.s64, sl.s32)
-                                       : bucket_sort_desc<SEL_NT, 8>(sbuf, m, sl.bins, sl.s64, sl.s32);
-    if (ok) return;
-  }
+  if (m > 64 && m <= BUCKET_SORT_MAX && bucket_sort_all(sbuf, m, sl)) return;
   const uint32_t P2 = next_pow2(m);
   for (uint32_t i = m + threadIdx.x; i < P2; i += SEL_NT) sbuf[i] = 0;
   __syncthreads();
   if (P2 > 1) block_bitonic_desc<SEL_NT>(sbuf, P2);
 }
 
-// Sort sbuf[0, m) (re-keyed) and write the row's output plus its tail fill.
+// Writes the row's output from sbuf[0, m), keys already in output order
+// (re-keyed), plus its tail fill.
 template <int DT, bool LARGEST>
-RK_DEV void sort_and_write(const KParams& p, const RowDesc& d, uint64_t* sbuf, uint32_t m, int mode,
-                           const SortLds& sl, int r = 0) {
+RK_DEV void write_row(const KParams& p, const RowDesc& d, const uint64_t* sbuf, uint32_t m,
+                      int mode) {
   using U = typename Traits<DT>::U;
-  sort_desc(sbuf, m, sl);
-  RK_STAMP(p, r, 13);
-  (void)r;
   const U* rb = static_cast<const U*>(d.base);
   U* ov = static_cast<U*>(d.out_v);
   for (uint32_t i = threadIdx.x; i < m; i += SEL_NT) {
@@ -999,6 +1000,16 @@ RK_DEV void sort_and_write(const KParams& p, const RowDesc& d, uint64_t* sbuf, u
     ov[i] = (U)0;
     put_idx(p, d, i, -1);
   }
+}
+
+// Sort sbuf[0, m) (re-keyed) and write the row's output plus its tail fill.
+template <int DT, bool LARGEST>
+RK_DEV void sort_and_write(const KParams& p, const RowDesc& d, uint64_t* sbuf, uint32_t m, int mode,
+                           const SortLds& sl, int r = 0) {
+  sort_desc(sbuf, m, sl);
+  RK_STAMP(p, r, 13);
+  (void)r;
+  write_row<DT, LARGEST>(p, d, sbuf, m, mode);
 }
 
 // ---------------------------------------------------------------------------
@@ -1021,11 +1032,12 @@ void k_select_finalize(KParams p) {
   extern __shared__ uint64_t sbuf[];
   __shared__ uint32_t hist[HIST_MAX_BINS];
   __shared__ uint32_t scratch[16];
-  __shared__ uint64_t scratch64[16];
+  __shared__ uint64_t scratch64[3 * SEL_NT / 64];
   __shared__ uint32_t outv[4];
   __shared__ uint32_t slot_cnt;
   __shared__ uint32_t next_sh;
-  const SortLds sl{hist, scratch64, scratch};
+  __shared__ uint32_t ctab[256];
+  const SortLds sl{hist, scratch64, scratch, ctab, (uint32_t)p.lds_keys};
   const int mode = order_mode(p.flags);
   for (;;) {
     if (threadIdx.x == 0) next_sh = atomicAdd(&p.hdr->q_select[p.round], 1u);
@@ -1121,6 +1133,28 @@ void k_select_finalize(KParams p) {
       }
       __syncthreads();
       RK_STAMP(p, r, 9);
+      // Candidates that one bucket sort takes (all of them >= T, at least k)
+      // are sorted whole: the first k are the selection, best first, so the
+      // radix select, the compaction and a second sort are skipped.
+      if (in_lds && n > 64 && n <= BUCKET_SORT_MAX && bucket_sort_all(sbuf, n, sl)) {
+        RK_STAMP(p, r, 10);
+        RK_STAMP(p, r, 11);
+        if (mode == ORD_BEST) {
+          RK_STAMP(p, r, 13);
+          write_row<DT, LARGEST>(p, d, sbuf, k, mode);
+        } else {
+          for (uint32_t i = threadIdx.x; i < k; i += SEL_NT) {
+            const uint64_t x = sbuf[i];
+            sbuf[i] = order_key<DT>(mode, (uint32_t)(x >> 32), ~(uint32_t)x);
+          }
+          sort_and_write<DT, LARGEST>(p, d, sbuf, k, mode, sl, r);
+        }
+        RK_STAMP(p, r, 12);
+        __syncthreads();
+        if (threadIdx.x == 0) p.st[r].phase = PH_DONE;
+        __syncthreads();
+        continue;
+      }
       const uint64_t kth =
           in_lds ? block_radix_select<SEL_NT, 11, uint64_t, false>(keys, n, k, hist, scratch, scratch64, outv)
                  : block_radix_select<SEL_NT, 11, uint64_t, true>(keys, n, k, hist, scratch, scratch64, outv);
@@ -1242,8 +1276,10 @@ constexpr int MS_E = MERGE_CHUNK / MS_NT;   // keys per thread (registers)
 template <int DT>
 __global__ __launch_bounds__(MS_NT) void k_merge_sort(KParams p) {
   __shared__ uint64_t xs[MERGE_CHUNK];
+  __shared__ uint64_t xt[MERGE_CHUNK];  // the bucket sort's grouping buffer
   __shared__ uint32_t bins[2048];
-  __shared__ uint64_t s64[MS_NT / 64];
+  __shared__ uint32_t ctab[256];
+  __shared__ uint64_t s64[3 * MS_NT / 64];
   __shared__ uint32_t s32[MS_NT / 64];
   const int cpr = merge_chunks(p);
   const int64_t items = (int64_t)p.R * cpr;
@@ -1262,7 +1298,7 @@ __global__ __launch_bounds__(MS_NT) void k_merge_sort(KParams p) {
     RK_STAMP(p, r, sb + 1);
     // Buckets and ranks when the keys spread over their top varying bits,
     // else the bitonic network in registers (zeros pad it: they sort last).
-    if (n > 64 && bucket_sort_desc<MS_NT, MS_E>(xs, n, bins, s64, s32)) {
+    if (n > 64 && bucket_sort_desc<MS_NT>(xs, xt, n, bins, ctab, s64, s32)) {
       for (uint32_t i = threadIdx.x; i < n; i += MS_NT) keys[i] = xs[i];
     } else {
       uint64_t v[MS_E];
@@ -1846,9 +1882,10 @@ __global__ __launch_bounds__(SEL_NT) void k_finalize_ordered(KParams p) {
   using U = typename Traits<DT>::U;
   extern __shared__ uint64_t sbuf[];
   __shared__ uint32_t bins[HIST_MAX_BINS];
-  __shared__ uint64_t s64[2 * SEL_NT / 64];
+  __shared__ uint64_t s64[3 * SEL_NT / 64];
   __shared__ uint32_t s32[SEL_NT / 64];
-  const SortLds sl{bins, s64, s32};
+  __shared__ uint32_t ctab[256];
+  const SortLds sl{bins, s64, s32, ctab, (uint32_t)p.lds_keys};
   const int mode = order_mode(p.flags);
   for (int r = blockIdx.x; r < p.R; r += gridDim.x) {
     if (p.st[r].phase != PH_ORDERED) continue;
