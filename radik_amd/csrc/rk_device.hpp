@@ -498,37 +498,31 @@ RK_DEV void block_find_crossing(const uint32_t* hist, int nbins, uint32_t k,
 // Workgroup radix select: returns a threshold K such that
 //   #{i : keys[i] >= K} >= k  and  #{i : keys[i] > K'} < k for K' >= K in
 // the selected bin, i.e. keys >= K are exactly the k largest when the keys
-// are distinct (composite keys), and at least the k largest otherwise.
-// `keys` points to global memory (GLOBAL) or LDS.  hist: LDS, 1 << HB words.
-// The scan starts below the common prefix of all keys (the key-space form
-// of RadiK's adaptive scaling): bits every key shares are never histogrammed.
-// Loads are issued UN at a time before their LDS atomics.
+// are distinct (composite keys), and at least the k largest otherwise;
+// out[3] receives #{i : keys[i] >= K}.  `keys` points to global memory
+// (GLOBAL) or LDS.  hist: LDS, 1 << HB words.  The scan starts below the
+// common prefix of all keys (the key-space form of RadiK's adaptive
+// scaling): bits every key shares are never histogrammed.  This form takes
+// the bits that vary (diff) and one key (ref) from its caller.  Loads are
+// issued UN at a time before their LDS atomics.
 template <int NT, int HB, typename KeyT, bool GLOBAL>
-RK_DEV KeyT block_radix_select(const KeyT* keys, uint32_t n, uint32_t k, uint32_t* hist,
-                               uint32_t* scratch, uint64_t* scratch64, uint32_t* out) {
+RK_DEV KeyT block_radix_select_from(const KeyT* keys, uint32_t n, uint32_t k, uint64_t diff,
+                                    KeyT ref, uint32_t* hist, uint32_t* scratch,
+                                    uint32_t* out) {
   constexpr int BITS = (int)sizeof(KeyT) * 8;
   constexpr int UN = 4;
   auto ld = [&](uint32_t i) -> KeyT {
     if constexpr (GLOBAL) return ldg(keys + i);
     else return ld_lds(keys + i);
   };
-  const KeyT k0 = ld(0);
-  uint64_t diff = 0;
-  for (uint32_t i0 = threadIdx.x; i0 < n; i0 += NT * UN) {
-    KeyT x[UN];
-#pragma unroll
-    for (int u = 0; u < UN; ++u) {
-      const uint32_t i = i0 + u * NT;
-      x[u] = i < n ? ld(i) : k0;
-    }
-#pragma unroll
-    for (int u = 0; u < UN; ++u) diff |= (uint64_t)(x[u] ^ k0);
+  if (diff == 0) {  // all keys equal: every key is >= ref
+    if (threadIdx.x == 0) out[3] = n;
+    __syncthreads();
+    return ref;
   }
-  diff = block_or64<NT>(diff, scratch64);
-  if (diff == 0) return k0;  // all keys equal: every key is >= k0
   int lo = 64 - __builtin_clzll(diff);  // number of low bits still unresolved
-  KeyT prefix = (lo >= BITS) ? (KeyT)0 : (KeyT)((k0 >> lo) << lo);
-  uint32_t rem = k;
+  KeyT prefix = (lo >= BITS) ? (KeyT)0 : (KeyT)((ref >> lo) << lo);
+  uint32_t rem = k, cnt = 0;
   while (lo > 0) {
     const int w = lo < HB ? lo : HB;
     const int shift = lo - w;
@@ -551,15 +545,47 @@ RK_DEV KeyT block_radix_select(const KeyT* keys, uint32_t n, uint32_t k, uint32_
     }
     __syncthreads();
     block_find_crossing<NT>(hist, (int)nb, rem, scratch, out);
-    const uint32_t b = out[0], above = out[1], cnt = out[2];
+    const uint32_t b = out[0], above = out[1];
+    cnt = out[2];
     __syncthreads();
-    if (b == 0xffffffffu) return prefix;  // cannot happen when n >= k
+    if (b == 0xffffffffu) {  // cannot happen when n >= k
+      if (threadIdx.x == 0) out[3] = 0;
+      __syncthreads();
+      return prefix;
+    }
     prefix |= (KeyT)b << shift;
     rem -= above;
     lo = shift;
     if (cnt == rem) break;  // the whole bin is taken
   }
+  // keys above the final bin: k - rem; keys in it (all >= the threshold): cnt
+  if (threadIdx.x == 0) out[3] = k - rem + cnt;
+  __syncthreads();
   return prefix;
+}
+
+template <int NT, int HB, typename KeyT, bool GLOBAL>
+RK_DEV KeyT block_radix_select(const KeyT* keys, uint32_t n, uint32_t k, uint32_t* hist,
+                               uint32_t* scratch, uint64_t* scratch64, uint32_t* out) {
+  constexpr int UN = 4;
+  auto ld = [&](uint32_t i) -> KeyT {
+    if constexpr (GLOBAL) return ldg(keys + i);
+    else return ld_lds(keys + i);
+  };
+  const KeyT k0 = ld(0);
+  uint64_t diff = 0;
+  for (uint32_t i0 = threadIdx.x; i0 < n; i0 += NT * UN) {
+    KeyT x[UN];
+#pragma unroll
+    for (int u = 0; u < UN; ++u) {
+      const uint32_t i = i0 + u * NT;
+      x[u] = i < n ? ld(i) : k0;
+    }
+#pragma unroll
+    for (int u = 0; u < UN; ++u) diff |= (uint64_t)(x[u] ^ k0);
+  }
+  diff = block_or64<NT>(diff, scratch64);
+  return block_radix_select_from<NT, HB, KeyT, GLOBAL>(keys, n, k, diff, k0, hist, scratch, out);
 }
 
 // Descending bitonic sort of P2 (power of two) 64-bit keys held in LDS.

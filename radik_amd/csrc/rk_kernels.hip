@@ -554,7 +554,7 @@ __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
     const RowDesc d = p.desc[r];
     const U* rb = static_cast<const U*>(d.base);
     const int64_t len = d.len;
-    const int64_t ge = p.tune.group_elems;
+    const int64_t ge = p.tune.group_elems;  // a multiple of 8, so of PACK
     int64_t ng = p.tune.sample_groups;
     const int64_t ng_cap = p.sample_keys / ge;      // runs that fit the LDS keys
     if (ng * ge * 64 > len) ng = len / (64 * ge);  // read about 1/64 of the row,
@@ -563,39 +563,59 @@ __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
     if (ng > ng_cap) ng = ng_cap;                   // and never beyond the LDS
     if (ng < 1) ng = 1;
     const uint32_t s = (uint32_t)(ng * ge);
-    // run g starts at floor(g * (len - ge) / (ng - 1)) (in double: no 64-bit
-    // integer division per key), clamped so that the run stays in the row
-    const double step = ng > 1 ? (double)(len - ge) / (double)(ng - 1) : 0.0;
-    const uint32_t ge32 = (uint32_t)ge;
-    constexpr int UN = 16;  // loads in flight per thread
-    for (uint32_t i0 = threadIdx.x; i0 < s; i0 += SEL_NT * UN) {
-      uint32_t u[UN];
+    // Runs on the row's 16-byte grid, read as whole vectors: run g starts at
+    // a0 + floor(g * (len - a0 - ge) / (ng - 1)) rounded down to a vector
+    // (in double: no 64-bit integer division per run), so the last run ends
+    // inside the row.  Sampled rows are far longer than a run.
+    constexpr int P = Traits<DT>::PACK;
+    const int64_t a0 = aligned_start<DT>(rb, 0);
+    const double step = ng > 1 ? (double)(len - a0 - ge) / (double)(ng - 1) : 0.0;
+    const uint32_t vpr = (uint32_t)(ge / P);  // vectors per run
+    const uint32_t nvec = (uint32_t)ng * vpr;
+    constexpr int UN = 8;  // 16-byte loads in flight per thread
+    // the bits that vary among the sampled keys: OR ^ AND, gathered while
+    // loading, so the radix select skips its own pass over them
+    uint32_t kor = 0, kand = 0xffffffffu;
+    for (uint32_t v0 = threadIdx.x; v0 < nvec; v0 += SEL_NT * UN) {
+      uint4 x[UN];
 #pragma unroll
       for (int q = 0; q < UN; ++q) {
-        const uint32_t i = i0 + q * SEL_NT;
-        const uint32_t g = i / ge32, j = i - g * ge32;
-        int64_t start = (int64_t)((double)g * step);
-        start = start < len - ge ? start : len - ge;
-        u[q] = i < s ? (uint32_t)ldg(rb + start + j) : 0u;
+        const uint32_t v = v0 + q * SEL_NT;
+        const uint32_t g = v / vpr, j = v - g * vpr;
+        const int64_t start = a0 + ((int64_t)((double)g * step) / P) * P;
+        x[q] = v < nvec ? ldg16<false>(rb + start + (int64_t)j * P) : make_uint4(0u, 0u, 0u, 0u);
       }
 #pragma unroll
       for (int q = 0; q < UN; ++q) {
-        const uint32_t i = i0 + q * SEL_NT;
-        if (i < s) keys[i] = sel_key<DT, LARGEST>(u[q]);
+        const uint32_t v = v0 + q * SEL_NT;
+        if (v >= nvec) continue;
+        const Vec<DT> e = unpack<DT>(x[q]);
+#pragma unroll
+        for (int c = 0; c < P; ++c) {
+          const uint32_t kk = sel_key<DT, LARGEST>(e.e[c]);
+          keys[v * P + c] = kk;
+          kor |= kk;
+          kand &= kk;
+        }
       }
     }
-    __syncthreads();
+    {
+      const uint64_t both = block_or64<SEL_NT>(((uint64_t)(~kand) << 32) | kor, scratch64);
+      kand = ~(uint32_t)(both >> 32);
+      kor = (uint32_t)both;
+    }  // its barriers also complete the keys
     RK_STAMP(p, r, 1);
     const float mu = (float)s * (float)d.k / (float)len;
     float rf = ceilf(mu + 4.0f * sqrtf(mu) + 2.0f);
     uint32_t rank = rf >= (float)s ? s : (uint32_t)rf;
     if (rank < 1) rank = 1;
-    uint32_t T = 0;
-    if (rank < s) T = block_radix_select<SEL_NT, 11, uint32_t, false>(keys, s, rank, hist, scratch, scratch64, outv);
+    uint32_t T = 0, ge_cnt = s;
+    if (rank < s) {
+      T = block_radix_select_from<SEL_NT, 11, uint32_t, false>(keys, s, rank, kor ^ kand, kand,
+                                                              hist, scratch, outv);
+      ge_cnt = outv[3];  // sampled keys >= T
+    }
     RK_STAMP(p, r, 2);
-    uint32_t ge_cnt = 0;
-    for (uint32_t i = threadIdx.x; i < s; i += SEL_NT) ge_cnt += (keys[i] >= T) ? 1u : 0u;
-    ge_cnt = block_sum<SEL_NT>(ge_cnt, scratch);
     // Expected candidates: the sampled fraction at or above T, scaled.
     const double est = (double)ge_cnt * (double)len / (double)s;
     const uint32_t why = (T == 0 || rank >= s) ? FB_SAMPLE_DEGENERATE

@@ -136,13 +136,16 @@ def c4(dtype: str = "f32", scale: int = 1, rows: int = 2048) -> Workload:
                     [Segment(0, rows, NORMAL, 0.0, 1.0)], dtype)
 
 
-def sample_positions(n: int, sample_groups: int = 1024, group_elems: int = 16) -> np.ndarray:
+def sample_positions(n: int, sample_groups: int = 1024, group_elems: int = 16, pack: int = 4,
+                     a0: int = 0) -> np.ndarray:
     """Row positions the sampled path reads for a row of n elements: runs of
-    group_elems keys starting at floor(g * (n - group_elems) / (ng - 1)), with
-    ng the number of runs the library derives from sample_groups and n (at
-    most 16384 keys, about 1/64 of the row, at least 64 runs).  Used to build
-    inputs that defeat the sample; the sampled path itself is exact whatever
-    the sample sees."""
+    group_elems keys read as 16-byte vectors (pack elements each), run g
+    starting at a0 + floor(g * (n - a0 - group_elems) / (ng - 1)) rounded down
+    to a multiple of pack, with a0 the row's first element on a 16-byte
+    boundary and ng the number of runs the library derives from sample_groups
+    and n (at most 16384 keys, about 1/64 of the row, at least 64 runs).
+    Used to build inputs that defeat the sample; the sampled path itself is
+    exact whatever the sample sees."""
     ge = group_elems
     keys = min(16384, max(sample_groups, 64) * ge)
     ng = sample_groups
@@ -150,8 +153,8 @@ def sample_positions(n: int, sample_groups: int = 1024, group_elems: int = 16) -
         ng = n // (64 * ge)
     ng = max(ng, min(n // (2 * ge), 64))
     ng = max(min(ng, keys // ge), 1)
-    step = (n - ge) / (ng - 1) if ng > 1 else 0.0
-    starts = np.minimum((np.arange(ng) * step).astype(np.int64), n - ge)
+    step = (n - a0 - ge) / (ng - 1) if ng > 1 else 0.0
+    starts = a0 + (np.arange(ng) * step).astype(np.int64) // pack * pack
     return (starts[:, None] + np.arange(ge)[None, :]).ravel()
 
 
@@ -179,11 +182,15 @@ def _apply_adversary(w: Workload, vals, off) -> None:
     import torch
 
     shift = 8.0 if w.largest else -8.0
+    pack = 4 if w.dtype == "f32" else 8  # elements per 16-byte vector
     for n in np.unique(w.lens):
+        # rows of equal length n start on 16-byte boundaries here (16 | n * size)
+        assert (int(n) * (4 if w.dtype == "f32" else 2)) % 16 == 0
         if w.adversary == "decoy":
-            pos = sample_positions(int(n), 1024, 16)
+            pos = sample_positions(int(n), 1024, 16, pack)
         else:
-            seen = np.union1d(sample_positions(int(n), 1024, 16), sample_positions(int(n), 512, 16))
+            seen = np.union1d(sample_positions(int(n), 1024, 16, pack),
+                              sample_positions(int(n), 512, 16, pack))
             pos = np.setdiff1d(np.arange(int(n)), seen)
         d_pos = torch.from_numpy(pos).to(vals.device)
         for r in np.nonzero(w.lens == n)[0]:
@@ -262,11 +269,14 @@ def materialize(w: Workload, device="cuda"):
     off = w.offsets()
     vals = torch.empty(max(w.total, 1), dtype=tdt, device=device)
     if w.dtype != "f32":
-        # 16-bit runs round the same fp32 values (round to nearest even)
-        f32 = materialize(Workload(w.name, w.lens, w.ks, w.largest, w.seed, w.segments, "f32",
-                                   w.adversary), device)
+        # 16-bit runs round the same fp32 values (round to nearest even); an
+        # adversary is placed after rounding, on the 16-bit sample's grid
+        f32 = materialize(Workload(w.name, w.lens, w.ks, w.largest, w.seed, w.segments, "f32"),
+                          device)
         vals.copy_(f32[: vals.numel()])
         del f32
+        if w.adversary:
+            _apply_adversary(w, vals, off)
         return vals
     for seg in w.segments:
         for r in range(seg.r0, seg.r1):

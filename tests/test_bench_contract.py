@@ -48,19 +48,22 @@ def test_percentile_is_nearest_rank():
 
 
 def test_sample_positions_cover_evenly_spaced_runs():
-    """The adversarial workloads' picture of the sample: ng runs of 16 keys,
-    the first at 0, the last ending at the row's end, spaced about
-    (n - 16) / (ng - 1) apart, at most 16384 keys and about 1/64 of a row."""
+    """The adversarial workloads' picture of the sample: ng runs of 16 keys on
+    the row's 16-byte vector grid (pack elements per vector, from a0), the
+    first at a0, within pack of g * (n - a0 - 16) / (ng - 1) + a0, the last
+    inside the row, at most 16384 keys and about 1/64 of a row."""
     sys.path.insert(0, ROOT)
     import radik_amd.workloads as W
     for n, sg, ng in ((1 << 20, 1024, 1024), (1 << 22, 512, 512), (1 << 22, 2048, 1024),
                       (1 << 16, 1024, 64), (5000, 1024, 64)):
-        p = W.sample_positions(n, sg, 16)
-        assert len(p) == ng * 16
-        runs = p.reshape(ng, 16)
-        assert (np.diff(runs, axis=1) == 1).all()
-        assert runs[0, 0] == 0 and runs[-1, -1] == n - 1
-        gaps = np.diff(runs[:, 0])
-        assert gaps.min() >= (n - 16) // (ng - 1) and gaps.max() <= (n - 16) // (ng - 1) + 1
+        for pack, a0 in ((4, 0), (8, 0), (4, 3), (8, 5)):
+            p = W.sample_positions(n, sg, 16, pack, a0)
+            assert len(p) == ng * 16
+            runs = p.reshape(ng, 16)
+            assert (np.diff(runs, axis=1) == 1).all()
+            assert runs[0, 0] == a0 and runs[-1, -1] <= n - 1
+            assert ((runs[:, 0] - a0) % pack == 0).all()
+            ideal = a0 + np.arange(ng) * ((n - a0 - 16) / (ng - 1))
+            assert (runs[:, 0] <= ideal + 1e-6).all() and (runs[:, 0] > ideal - pack).all()
     w = W.adversarial("hidden", rows=2, n=1 << 20)
     assert w.adversary == "hidden" and w.rows == 2 and int(w.ks[0]) == 1024
