@@ -909,13 +909,30 @@ __global__ __launch_bounds__(STREAM_NT)
 __attribute__((amdgpu_waves_per_eu(U >= 8 ? 1 : (U == 4 ? 4 : 7))))
 void k_filter(KParams p) {
   __shared__ FilterLds lds;
+#ifdef RK_PHASE_TIMING
+  // per-workgroup start, end and segment count (tuning builds), in the
+  // histogram slots of row blockIdx.x when the call has that many rows
+  uint32_t nseg = 0;
+  const uint64_t t_start = wall_clock64();
+#endif
   for_each_chunk_queued(p, &p.hdr->q_chunk[0], [&](int r, uint32_t ta, uint32_t tb) {
     // the row's metadata, loaded together (one round trip)
     const RowState st = p.st[r];
     const RowDesc d = p.desc[r];
     const uint64_t cbase = p.cand_base[r];
     filter_segment<DT, LARGEST, U>(p, lds, st, d, cbase, r, ta, tb);
+#ifdef RK_PHASE_TIMING
+    ++nseg;
+#endif
   });
+#ifdef RK_PHASE_TIMING
+  if (threadIdx.x == 0 && (int)blockIdx.x < p.R) {
+    uint32_t* w = p.ghist + (size_t)blockIdx.x * HIST_MAX_BINS + 1024;
+    w[0] = (uint32_t)t_start;
+    w[1] = (uint32_t)wall_clock64();
+    w[2] = nseg;
+  }
+#endif
 }
 
 // ---------------------------------------------------------------------------
