@@ -117,10 +117,11 @@ Tune current_tune(const DevInfo& di, int dtype, uint32_t flags, int32_t rows, in
   if (t.tile < rk::TILE_MIN) t.tile = rk::TILE_MIN;
   t.tile = (t.tile / 64) * 64;
   // Default grid: the sampled path's filter keeps two load sets in flight
-  // per wave and runs best at 4 workgroups per CU; the exact path's
-  // histogram passes at 8.
+  // per wave and runs best at 4 workgroups per CU; the exact-pass executor
+  // at 1 (2 from 2^26 keys), where its chunks' claims, histogram merges and
+  // completions (device-scope round trips) contend least.
   const bool exact = (flags & RK_EXACT) != 0;
-  if (t.grid <= 0) t.grid = di.cus * (exact ? 8 : 4);
+  if (t.grid <= 0) t.grid = di.cus * (exact ? rk::exact_grid_per_cu(total) : 4);
   if (t.sample_groups <= 0) t.sample_groups = 1024;
   if (t.group_elems <= 0) t.group_elems = 16;
   t.group_elems = std::min(256, std::max(8, (t.group_elems / 8) * 8));
@@ -225,8 +226,19 @@ int launch(int dtype, const void* vals, const int64_t* row_off, const void* cons
   int64_t ct = std::max<int64_t>(1, chunk_bytes / ((int64_t)p.tune.tile * es));
   while (ct > 1 && (int64_t)tiles_bound / ct < 2 * (int64_t)grid) ct >>= 1;
   p.chunk_tiles = (int32_t)ct;
-  // The exact-pass executor hands rows out in chunks of at least 64K keys.
-  p.x_chunk_tiles = std::max(1, 65536 / p.tune.tile);
+  // The exact-pass executor hands rows out in chunks of about two per
+  // workgroup of the call's whole work, within [16K, 1M] keys: each chunk
+  // pays a few device-scope round trips (claim, histogram merge,
+  // completion), which larger chunks amortise, and a single row still
+  // splits into many.
+  {
+    const int64_t xg = (flags & RK_EXACT)
+                           ? std::max<int64_t>(1, std::min<int64_t>(grid, (int64_t)di.cus * 4))
+                           : (int64_t)di.cus * rk::exact_grid_per_cu(total_len);
+    const int64_t xe = std::min<int64_t>(int64_t(1) << 20,
+                                         std::max<int64_t>(16384, total_len / (2 * xg)));
+    p.x_chunk_tiles = (int32_t)std::max<int64_t>(1, (xe + p.tune.tile - 1) / p.tune.tile);
+  }
   const hipError_t e =
       rk::run_pipeline(dtype, p, static_cast<hipStream_t>(stream), row_grid, grid, part, publish);
   return e == hipSuccess ? RK_OK : RK_E_HIP;

@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -411,6 +412,9 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
     h->q_chunk[1] = 0;
     h->x_rows = 0;
     h->x_left = 0;
+#ifdef RK_PHASE_TIMING
+    for (int i = 30; i < 50; ++i) h->pad[i] = 0;
+#endif
     if (carry_t > p.tile_capacity) status |= ST_TILES;
     if (carry_c > p.cand_capacity) status |= ST_CANDIDATES;
     h->status = status;
@@ -498,11 +502,11 @@ RK_DEV void exact_start_row(const KParams& p, int r, uint32_t why) {
     w.why = why;
     w.phase = PH_HIST;
     p.cand_cnt[r] = 0;
-    // chunks of up to x_chunk_tiles tiles, but at least 64 per row (when it
-    // has that many tiles), so that a single row still spreads over the grid
+    // chunks of x_chunk_tiles tiles (sized by the host from the call's work)
     const uint32_t tiles = p.tile_start[r + 1] - p.tile_start[r];
-    uint32_t ct = tiles / 64;
-    ct = ct < 1 ? 1 : (ct > (uint32_t)p.x_chunk_tiles ? (uint32_t)p.x_chunk_tiles : ct);
+    uint32_t ct = (uint32_t)p.x_chunk_tiles;
+    ct = ct > tiles ? tiles : ct;
+    ct = ct < 1 ? 1 : ct;
     XRow& x = p.xst[r];
     x.claim = 0;
     x.done = 0;
@@ -710,6 +714,39 @@ RK_DEV void append_vec(const AppendCtx& ac, uint32_t& n, bool& full, uint4 x, ui
   n += tot;
 }
 
+// The vector grid of a row, shared by every streaming pass: 16-byte vectors
+// counted from a0, the row's first element on a 128-byte boundary (an L2
+// line), so a wave's 1 KB load covers 8 lines, not 9 (rows start anywhere;
+// measured on C4: +2 % fp32, +3 % bf16).  A segment of tiles [ta, tb) owns
+// vectors [ta * tile / P, tb * tile / P) of that grid; the row's first
+// segment also owns the head elements [0, a0) (fewer than 128 bytes) and its
+// last segment the tail elements after the last whole vector (fewer than P).
+struct SegGrid {
+  int64_t a0;     // the row's first grid element
+  int64_t al;     // first element of the segment's vectors
+  int64_t nv;     // vectors of the segment
+  int64_t tail0;  // first element after the row's last whole vector
+  bool first_seg, last_seg;
+};
+template <int DT>
+RK_DEV SegGrid seg_grid(const void* base, int64_t len, int tile, uint32_t ta, uint32_t tb) {
+  constexpr int P = Traits<DT>::PACK;
+  constexpr int ESZ = (int)sizeof(typename Traits<DT>::U);
+  SegGrid g;
+  const int64_t a_line = (int64_t)(((128u - ((uint64_t)base & 127u)) & 127u) / ESZ);
+  g.a0 = a_line < len ? a_line : len;
+  const int64_t nv_row = (len - g.a0) / P;
+  const int64_t v_lo = (int64_t)ta * (tile / P);
+  int64_t v_hi = (int64_t)tb * (tile / P);
+  if (v_hi > nv_row) v_hi = nv_row;
+  g.al = g.a0 + (v_lo < v_hi ? v_lo : v_hi) * P;
+  g.nv = v_hi > v_lo ? v_hi - v_lo : 0;
+  g.tail0 = g.a0 + nv_row * P;
+  g.first_seg = (ta == 0);
+  g.last_seg = (int64_t)tb * tile >= len;
+  return g;
+}
+
 // LDS of one streaming workgroup's candidate staging.
 struct FilterLds {
   uint64_t stage[STREAM_NT / 64][WSTAGE];
@@ -742,25 +779,9 @@ RK_DEV void filter_segment(const KParams& p, FilterLds& lds, const RowState& st,
     const bool all = (T == 0);
     const uint32_t xm = prefilter_mask<DT, LARGEST>(T);
     const UT* rb = static_cast<const UT*>(d.base);
-    // 16-byte vectors are counted from a0, the row's first element on a
-    // 128-byte boundary (an L2 line): a wave's 1 KB load then covers 8
-    // lines, not 9 (rows start anywhere; measured on C4: +2% fp32, +3%
-    // bf16).  The segment owns vectors [ta * tile / P, tb * tile / P) of that
-    // grid, so only the row's first segment has head elements [0, a0) (at
-    // most 63) and only its last segment the tail elements after the last
-    // whole vector.
-    constexpr int ESZ = (int)sizeof(UT);
-    const int64_t a_line = (int64_t)(((128u - ((uint64_t)rb & 127u)) & 127u) / ESZ);
-    const int64_t a0 = a_line < d.len ? a_line : d.len;
-    const int64_t nv_row = (d.len - a0) / P;
-    const int64_t v_lo = (int64_t)ta * (p.tune.tile / P);
-    int64_t v_hi = (int64_t)tb * (p.tune.tile / P);
-    if (v_hi > nv_row) v_hi = nv_row;
-    const int64_t al = a0 + (v_lo < v_hi ? v_lo : v_hi) * P;  // first element of the body
-    const int64_t nv = v_hi > v_lo ? v_hi - v_lo : 0;
-    const bool first_seg = (ta == 0);
-    const bool last_seg = (int64_t)tb * p.tune.tile >= d.len;
-    const int64_t tail0 = a0 + nv_row * P;
+    const SegGrid g = seg_grid<DT>(d.base, d.len, p.tune.tile, ta, tb);
+    const int64_t a0 = g.a0, al = g.al, nv = g.nv, tail0 = g.tail0;
+    const bool first_seg = g.first_seg, last_seg = g.last_seg;
 
     auto pred = [&](uint32_t u) -> bool { return all || sel_key<DT, LARGEST>(u) >= T; };
     AppendCtx ac;
@@ -1470,8 +1491,11 @@ struct HistLds {
 
 // Digit histogram of tiles [ta, tb) of PH_HIST row r added to the row's
 // global histogram: per-wave LDS sub-histograms (one atomic per wave when
-// all its active lanes hit one bin, as on skewed rows), merged with one
-// global atomic per non-empty bin.  Called by a whole workgroup.
+// all its counted lanes hit one bin, as on skewed rows), merged with one
+// global atomic per non-empty bin.  The row's vector grid is streamed with
+// buffer loads, software pipelined as in the filter; the loads keep the
+// lines cached, since the row's next pass reads them again.  Called by a
+// whole workgroup.
 template <int DT, bool LARGEST, int U>
 RK_DEV void hist_segment(const KParams& p, HistLds& lds, const RowState& st, const RowDesc& d,
                          int r, uint32_t ta, uint32_t tb) {
@@ -1479,11 +1503,17 @@ RK_DEV void hist_segment(const KParams& p, HistLds& lds, const RowState& st, con
   constexpr int P = Traits<DT>::PACK;
   constexpr int KB = Traits<DT>::KEY_BITS;
   constexpr int NW = STREAM_NT / 64;
+#ifdef RK_PHASE_TIMING
+  const uint64_t hs0 = wall_clock64();
+#endif
   for (int i = threadIdx.x; i < NW * HIST_MAX_BINS; i += STREAM_NT) (&lds.sub[0][0])[i] = 0;
   __syncthreads();
+#ifdef RK_PHASE_TIMING
+  const uint64_t hs1 = wall_clock64();
+#endif
   const int lane = lane_id();
-  const int wave_base = threadIdx.x & ~63;
-  uint32_t* h = lds.sub[threadIdx.x >> 6];
+  const int wave_base = __builtin_amdgcn_readfirstlane(threadIdx.x & ~63);
+  uint32_t* h = lds.sub[wave_base >> 6];
   const int HB = p.tune.hist_bits;
   const uint32_t b = st.resolved;
   const uint32_t P0 = st.prefix;
@@ -1491,125 +1521,147 @@ RK_DEV void hist_segment(const KParams& p, HistLds& lds, const RowState& st, con
   const int shift = KB - (int)b - w;
   const uint32_t nb = 1u << w;
   const UT* rb = static_cast<const UT*>(d.base);
-  const int64_t es = (int64_t)ta * p.tune.tile;
-  int64_t ee = (int64_t)tb * p.tune.tile;
-  if (ee > d.len) ee = d.len;
-  int64_t al = aligned_start<DT>(rb, es);
-  if (al > ee) al = ee;
-  const int64_t nv = (ee - al) / P;
-  const int64_t tail0 = al + nv * P;
+  const SegGrid g = seg_grid<DT>(d.base, d.len, p.tune.tile, ta, tb);
 
+  // One key: counted when it carries the resolved prefix.  Within the lanes
+  // that count it, the first one adds for all of them when they share a
+  // bin (readfirstlane / ballot, no lane exchange through LDS).
   auto count = [&](uint32_t u, bool valid) {
     const uint32_t sk = sel_key<DT, LARGEST>(u);
     const bool ok = valid && (b == 0 || (uint32_t)((uint64_t)sk >> (KB - b)) == P0);
-    const uint32_t dg = (sk >> shift) & (nb - 1);
-    const uint64_t act = __ballot(ok);
-    if (act) {
-      const int fl = __ffsll((unsigned long long)act) - 1;
-      const uint32_t d0 = __shfl(dg, fl, 64);
-      const uint64_t same = __ballot(ok && dg == d0);
+    if (ok) {
+      const uint32_t dg = (sk >> shift) & (nb - 1);
+      const uint32_t d0 = __builtin_amdgcn_readfirstlane(dg);
+      const uint64_t act = __ballot(true);
+      const uint64_t same = __ballot(dg == d0);
       if (same == act) {
-        if (lane == fl) atomicAdd(&h[d0], (uint32_t)__popcll(act));
-      } else if (ok) {
+        if (mbcnt(act) == 0) atomicAdd(&h[d0], (uint32_t)__popcll(act));
+      } else {
         atomicAdd(&h[dg], 1u);
       }
     }
   };
 
-  if (wave_base == 0) {
+  // the row's head and tail elements outside the vector grid, one per lane
+  if ((wave_base == 0 && g.first_seg) || (wave_base == STREAM_NT - 64 && g.last_seg)) {
     int64_t pos = -1;
-    if (lane < P && es + lane < al) pos = es + lane;
-    if (lane >= 32 && lane - 32 < P && tail0 + (lane - 32) < ee) pos = tail0 + (lane - 32);
-    const uint32_t u = pos >= 0 ? (uint32_t)rb[pos] : 0u;
-    count(u, pos >= 0);
+    if (wave_base == 0 && g.first_seg && lane < g.a0) pos = lane;
+    if (wave_base != 0 && lane < P && g.tail0 + lane < d.len) pos = g.tail0 + lane;
+    count(pos >= 0 ? (uint32_t)rb[pos] : 0u, pos >= 0);
   }
-  const uint4* vp = reinterpret_cast<const uint4*>(rb + al);
-  for (int64_t jb = wave_base; jb < nv; jb += (int64_t)STREAM_NT * U) {
-    uint4 x[U];
+  const uint4* vp = reinterpret_cast<const uint4*>(rb + g.al);
+  constexpr uint32_t STEP = (uint32_t)STREAM_NT * U;
+  constexpr int64_t CHUNK = int64_t(1) << 26;  // vectors per descriptor (32-bit offsets)
+  for (int64_t c0 = 0; c0 < g.nv; c0 += CHUNK) {
+    const uint32_t nvc = (uint32_t)((g.nv - c0) < CHUNK ? (g.nv - c0) : CHUNK);
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(vp + c0, nvc * 16u);
+    auto load_set = [&](uint4 (&x)[U], uint32_t jb) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int64_t j = jb + lane + (int64_t)u * STREAM_NT;
-      x[u] = j < nv ? ldg16<false>(vp + j) : make_uint4(0u, 0u, 0u, 0u);
-    }
+      for (int u = 0; u < U; ++u) x[u] = bload16(rs, (jb + lane + u * STREAM_NT) * 16u);
+    };
+    uint4 x[U], y[U];
+    uint32_t jb = (uint32_t)wave_base;
+    if (jb < nvc) load_set(x, jb);
+    for (; jb < nvc; jb += STEP) {  // wave-uniform
+      load_set(y, jb + STEP);
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int64_t j = jb + lane + (int64_t)u * STREAM_NT;
-      const Vec<DT> v = unpack<DT>(x[u]);
+      for (int u = 0; u < U; ++u) {
+        const bool valid = jb + lane + u * STREAM_NT < nvc;
+        const Vec<DT> v = unpack<DT>(x[u]);
 #pragma unroll
-      for (int c = 0; c < P; ++c) count(v.e[c], j < nv);
+        for (int c = 0; c < P; ++c) count(v.e[c], valid);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) x[u] = y[u];
     }
   }
   __syncthreads();
-  uint32_t* g = p.ghist + (size_t)r * HIST_MAX_BINS;
+#ifdef RK_PHASE_TIMING
+  const uint64_t hs2 = wall_clock64();
+  uint32_t nz = 0;
+#endif
+  uint32_t* gh = p.ghist + (size_t)r * HIST_MAX_BINS;
   for (uint32_t i = threadIdx.x; i < nb; i += STREAM_NT) {
     uint32_t s = 0;
 #pragma unroll
     for (int q = 0; q < NW; ++q) s += lds.sub[q][i];
-    if (s) atomicAdd(&g[i], s);
+    if (s) atomicAdd(&gh[i], s);
+#ifdef RK_PHASE_TIMING
+    nz += s ? 1u : 0u;
+#endif
   }
+#ifdef RK_PHASE_TIMING
+  wait_vmem();
+  __syncthreads();
+  const uint64_t hs3 = wall_clock64();
+  nz = block_sum<STREAM_NT>(nz, reinterpret_cast<uint32_t*>(&lds.sub[0][0]));
+  if (threadIdx.x == 0) {
+    uint32_t* xt = p.hdr->pad;
+    atomicAdd(&xt[40], (uint32_t)(hs1 - hs0));
+    atomicAdd(&xt[41], (uint32_t)(hs2 - hs1));
+    atomicAdd(&xt[42], (uint32_t)(hs3 - hs2));
+    atomicAdd(&xt[43], nz);
+    atomicAdd(&xt[44], 1u);
+  }
+  __syncthreads();
+#endif
 }
 
 // Ordered emit for rows whose ties at the k-th key V exceed the candidate
-// slots (e.g. an all-equal row).  Counting pass: per chunk, the keys > V
-// and == V; the pass's last workgroup turns the chunk counts into prefix
-// sums.  Writing pass: each chunk writes, in row-position order, its keys
-// > V and those == V whose rank among equals is below rem, at output
-// position gt_before + min(rem, eq_before).
-template <int DT>
-RK_DEV uint32_t load_slot(const typename Traits<DT>::U* rb, int64_t es, int64_t al, int64_t nv,
-                          int64_t ee, int64_t s, Vec<DT>& v, int64_t& first) {
-  constexpr int P = Traits<DT>::PACK;
-  if (s >= 1 && s <= nv) {
-    first = al + (s - 1) * P;
-    v = unpack<DT>(ldg16<false>(rb + first));
-    return (1u << P) - 1u;
-  }
-  int64_t lo, hi;
-  if (s == 0) {
-    lo = es;
-    hi = al < ee ? al : ee;
-  } else {
-    lo = al + nv * P;
-    hi = ee;
-  }
-  first = lo;
-  uint32_t mask = 0;
-#pragma unroll
-  for (int c = 0; c < P; ++c) {
-    const bool ok = lo + c < hi;
-    v.e[c] = ok ? (uint32_t)rb[lo + c] : 0u;
-    mask |= (ok ? 1u : 0u) << c;
-  }
-  return mask;
-}
-
-template <int DT, bool LARGEST>
+// slots (e.g. an all-equal row), and unsorted rows with k above the LDS
+// select.  Counting pass: per chunk, the keys > V and == V; the pass's last
+// workgroup turns the chunk counts into prefix sums.  Writing pass: each
+// chunk writes, in row-position order, its keys > V and those == V whose
+// rank among equals is below rem, at output position gt_before +
+// min(rem, eq_before).  A chunk is a segment of the row's vector grid (its
+// head and tail elements in the first and last chunk), the same elements in
+// both passes.
+template <int DT, bool LARGEST, int U>
 RK_DEV void emit_count_chunk(const KParams& p, const RowState& st, const RowDesc& d, int r,
                              uint32_t c, uint32_t ta, uint32_t tb, uint32_t* scratch) {
   using UT = typename Traits<DT>::U;
   constexpr int P = Traits<DT>::PACK;
   const uint32_t V = st.V;
   const UT* rb = static_cast<const UT*>(d.base);
+  const SegGrid g = seg_grid<DT>(d.base, d.len, p.tune.tile, ta, tb);
+  const int lane = lane_id();
+  const int wave_base = __builtin_amdgcn_readfirstlane(threadIdx.x & ~63);
   uint32_t gt = 0, eq = 0;
-  for (uint32_t t = ta; t < tb; ++t) {
-    const int64_t es = (int64_t)t * p.tune.tile;
-    int64_t ee = es + p.tune.tile;
-    if (ee > d.len) ee = d.len;
-    int64_t al = aligned_start<DT>(rb, es);
-    if (al > ee) al = ee;
-    const int64_t nv = (ee - al) / P;
-    for (int64_t s = threadIdx.x; s < nv + 2; s += STREAM_NT) {
-      Vec<DT> v;
-      int64_t first;
-      const uint32_t mask = load_slot<DT>(rb, es, al, nv, ee, s, v, first);
+  auto tally = [&](uint32_t u, bool valid) {
+    const uint32_t sk = sel_key<DT, LARGEST>(u);
+    gt += (valid && sk > V) ? 1u : 0u;
+    eq += (valid && sk == V) ? 1u : 0u;
+  };
+  if ((wave_base == 0 && g.first_seg) || (wave_base == STREAM_NT - 64 && g.last_seg)) {
+    int64_t pos = -1;
+    if (wave_base == 0 && g.first_seg && lane < g.a0) pos = lane;
+    if (wave_base != 0 && lane < P && g.tail0 + lane < d.len) pos = g.tail0 + lane;
+    tally(pos >= 0 ? (uint32_t)rb[pos] : 0u, pos >= 0);
+  }
+  const uint4* vp = reinterpret_cast<const uint4*>(rb + g.al);
+  constexpr uint32_t STEP = (uint32_t)STREAM_NT * U;
+  constexpr int64_t CHUNK = int64_t(1) << 26;
+  for (int64_t c0 = 0; c0 < g.nv; c0 += CHUNK) {
+    const uint32_t nvc = (uint32_t)((g.nv - c0) < CHUNK ? (g.nv - c0) : CHUNK);
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(vp + c0, nvc * 16u);
+    auto load_set = [&](uint4 (&x)[U], uint32_t jb) {
 #pragma unroll
-      for (int e = 0; e < P; ++e) {
-        if ((mask >> e) & 1u) {
-          const uint32_t sk = sel_key<DT, LARGEST>(v.e[e]);
-          gt += sk > V ? 1u : 0u;
-          eq += sk == V ? 1u : 0u;
-        }
+      for (int u = 0; u < U; ++u) x[u] = bload16(rs, (jb + lane + u * STREAM_NT) * 16u);
+    };
+    uint4 x[U], y[U];
+    uint32_t jb = (uint32_t)wave_base;
+    if (jb < nvc) load_set(x, jb);
+    for (; jb < nvc; jb += STEP) {
+      load_set(y, jb + STEP);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const bool valid = jb + lane + u * STREAM_NT < nvc;
+        const Vec<DT> v = unpack<DT>(x[u]);
+#pragma unroll
+        for (int e = 0; e < P; ++e) tally(v.e[e], valid);
       }
+#pragma unroll
+      for (int u = 0; u < U; ++u) x[u] = y[u];
     }
   }
   gt = block_sum<STREAM_NT>(gt, scratch);
@@ -1619,7 +1671,7 @@ RK_DEV void emit_count_chunk(const KParams& p, const RowState& st, const RowDesc
            ((uint64_t)eq << 32) | gt);
 }
 
-template <int DT, bool LARGEST>
+template <int DT, bool LARGEST, int U>
 RK_DEV void emit_write_chunk(const KParams& p, const RowState& st, const RowDesc& d, int r,
                              uint32_t c, uint32_t ta, uint32_t tb, uint32_t* scratch) {
   using UT = typename Traits<DT>::U;
@@ -1629,60 +1681,102 @@ RK_DEV void emit_write_chunk(const KParams& p, const RowState& st, const RowDesc
   const uint32_t kk = (uint32_t)d.k;
   const UT* rb = static_cast<const UT*>(d.base);
   UT* ov = static_cast<UT*>(d.out_v);
+  const SegGrid g = seg_grid<DT>(d.base, d.len, p.tune.tile, ta, tb);
   // keys > V and == V in the row's chunks before this one
   const uint64_t pre = ld_dev(reinterpret_cast<const uint64_t*>(p.tile_cnt + p.tile_start[r] + c));
   uint32_t run_g = (uint32_t)pre, run_e = (uint32_t)(pre >> 32);
-  for (uint32_t t = ta; t < tb; ++t) {
-    const int64_t es = (int64_t)t * p.tune.tile;
-    int64_t ee = es + p.tune.tile;
-    if (ee > d.len) ee = d.len;
-    int64_t al = aligned_start<DT>(rb, es);
-    if (al > ee) al = ee;
-    const int64_t nv = (ee - al) / P;
-    for (int64_t s0 = 0; s0 < nv + 2; s0 += STREAM_NT) {
-      const int64_t s = s0 + threadIdx.x;
-      Vec<DT> v;
-      int64_t first = 0;
-      uint32_t mask = 0;
-      if (s < nv + 2) mask = load_slot<DT>(rb, es, al, nv, ee, s, v, first);
-      uint32_t g = 0, q = 0;
-      uint32_t sk[P];
+  // One block step: thread t holds n elements at row positions first..,
+  // consecutive in the row across threads (t, then e); keys > V and the
+  // first rem keys == V are written at their ranks.
+  // (n <= NMAX; the element loops are unrolled over NMAX, so the array
+  // stays in registers)
+  auto step = [&](auto nmax_tag, const uint32_t* u, uint32_t n, int64_t first) {
+    constexpr int NMAX = decltype(nmax_tag)::value;
+    uint32_t gq = 0;
 #pragma unroll
-      for (int e = 0; e < P; ++e) {
-        sk[e] = ((mask >> e) & 1u) ? sel_key<DT, LARGEST>(v.e[e]) : 0u;
-        const bool ok = (mask >> e) & 1u;
-        g += (ok && sk[e] > V) ? 1u : 0u;
-        q += (ok && sk[e] == V) ? 1u : 0u;
-      }
-      uint32_t tot;
-      const uint32_t ex = block_excl_scan<STREAM_NT>(g | (q << 16), scratch, tot);
-      uint32_t gb = run_g + (ex & 0xffffu);
-      uint32_t qb = run_e + (ex >> 16);
-#pragma unroll
-      for (int e = 0; e < P; ++e) {
-        if (!((mask >> e) & 1u)) continue;
-        const uint32_t pos = (uint32_t)(first + e);
-        if (sk[e] > V) {
-          const uint32_t o = gb + (qb < rem ? qb : rem);
-          if (o < kk) {
-            ov[o] = (UT)v.e[e];
-            put_idx(p, d, o, pos);  // a position: mapped by k_finalize_ordered
-          }
-          ++gb;
-        } else if (sk[e] == V) {
-          if (qb < rem) {
-            const uint32_t o = gb + qb;
-            if (o < kk) {
-              ov[o] = (UT)v.e[e];
-              put_idx(p, d, o, pos);
-            }
-          }
-          ++qb;
-        }
-      }
-      run_g += tot & 0xffffu;
-      run_e += tot >> 16;
+    for (int e = 0; e < NMAX; ++e) {
+      const uint32_t sk = sel_key<DT, LARGEST>(u[e]);
+      gq += (uint32_t)e >= n ? 0u : (sk > V ? 1u : (sk == V ? 0x10000u : 0u));
     }
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<STREAM_NT>(gq, scratch, tot);
+    uint32_t gb = run_g + (ex & 0xffffu);
+    uint32_t qb = run_e + (ex >> 16);
+#pragma unroll
+    for (int e = 0; e < NMAX; ++e) {
+      if ((uint32_t)e >= n) break;
+      const uint32_t sk = sel_key<DT, LARGEST>(u[e]);
+      const uint32_t pos = (uint32_t)(first + e);
+      if (sk > V) {
+        const uint32_t o = gb + (qb < rem ? qb : rem);
+        if (o < kk) {
+          ov[o] = (UT)u[e];
+          put_idx(p, d, o, pos);  // a position: mapped by k_finalize_ordered
+        }
+        ++gb;
+      } else if (sk == V) {
+        if (qb < rem) {
+          const uint32_t o = gb + qb;
+          if (o < kk) {
+            ov[o] = (UT)u[e];
+            put_idx(p, d, o, pos);
+          }
+        }
+        ++qb;
+      }
+    }
+    run_g += tot & 0xffffu;
+    run_e += tot >> 16;
+  };
+  // head elements (first chunk): one per thread, before every vector
+  if (g.first_seg) {
+    const int64_t pos = threadIdx.x;
+    uint32_t u = 0;
+    const uint32_t n = pos < g.a0 ? 1u : 0u;
+    if (n) u = (uint32_t)rb[pos];
+    step(std::integral_constant<int, 1>{}, &u, n, pos);
+  }
+  // body: UW consecutive vectors per thread (position order is
+  // thread-major within a block step; at most 4, so that the step's
+  // elements stay in registers); the next step's loads are issued before
+  // this one is written
+  constexpr int UW = U < 4 ? U : 4;
+  const uint4* vp = reinterpret_cast<const uint4*>(rb + g.al);
+  constexpr uint32_t BLK = (uint32_t)STREAM_NT * UW;
+  constexpr int64_t CHUNK = int64_t(1) << 26;
+  for (int64_t c0 = 0; c0 < g.nv; c0 += CHUNK) {
+    const uint32_t nvc = (uint32_t)((g.nv - c0) < CHUNK ? (g.nv - c0) : CHUNK);
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(vp + c0, nvc * 16u);
+    auto load_set = [&](uint4 (&x)[UW], uint32_t jb) {
+#pragma unroll
+      for (int u = 0; u < UW; ++u) x[u] = bload16(rs, (jb + threadIdx.x * UW + u) * 16u);
+    };
+    uint4 x[UW], y[UW];
+    uint32_t jb = 0;
+    load_set(x, jb);
+    for (; jb < nvc; jb += BLK) {  // workgroup-uniform
+      load_set(y, jb + BLK);
+      const uint32_t j0 = jb + threadIdx.x * UW;
+      const uint32_t nvec = j0 >= nvc ? 0u : (nvc - j0 < (uint32_t)UW ? nvc - j0 : (uint32_t)UW);
+      uint32_t u[UW * P];
+#pragma unroll
+      for (int q = 0; q < UW; ++q) {
+        const Vec<DT> v = unpack<DT>(x[q]);
+#pragma unroll
+        for (int e = 0; e < P; ++e) u[q * P + e] = v.e[e];
+      }
+      step(std::integral_constant<int, UW * P>{}, u, nvec * P, g.al + (int64_t)(c0 + j0) * P);
+#pragma unroll
+      for (int q = 0; q < UW; ++q) x[q] = y[q];
+    }
+  }
+  // tail elements (last chunk): after every vector
+  if (g.last_seg) {
+    const int64_t pos = g.tail0 + threadIdx.x;
+    uint32_t u = 0;
+    const uint32_t n = pos < d.len ? 1u : 0u;
+    if (n) u = (uint32_t)rb[pos];
+    step(std::integral_constant<int, 1>{}, &u, n, pos);
   }
 }
 
@@ -1698,9 +1792,21 @@ RK_DEV bool finish_hist(const KParams& p, int r, const RowState& s, uint32_t* hi
   const int w = (KB - b) < HB ? (KB - b) : HB;
   const uint32_t nb = 1u << w;
   uint32_t* g = p.ghist + (size_t)r * HIST_MAX_BINS;
-  for (uint32_t i = threadIdx.x; i < nb; i += STREAM_NT) {
-    hist[i] = ld_dev(g + i);
-    st_dev(g + i, 0u);
+  // all of a thread's bins loaded before any is used: one round trip
+  constexpr int PER = HIST_MAX_BINS / STREAM_NT;
+  uint32_t hv[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const uint32_t i = threadIdx.x + (uint32_t)j * STREAM_NT;
+    hv[j] = i < nb ? ld_dev(g + i) : 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const uint32_t i = threadIdx.x + (uint32_t)j * STREAM_NT;
+    if (i < nb) {
+      hist[i] = hv[j];
+      st_dev(g + i, 0u);
+    }
   }
   __syncthreads();
   block_find_crossing<STREAM_NT>(hist, (int)nb, s.rem, scratch, outv);
@@ -1784,6 +1890,29 @@ void k_exact(KParams p) {
   const int tid = threadIdx.x;
   uint32_t cursor = blockIdx.x % nrows;
   uint64_t wait0 = 0;
+  uint32_t idle_naps = 1;
+  RowState st;  // pass state of row st_r in pass st_seq
+  int st_r = -1;
+  uint32_t st_seq = 0;
+#ifdef RK_PHASE_TIMING
+  // tuning builds: the header's spare words hold workgroup 0's entry and
+  // exit and, for the first listed row, each stage's first claim and end
+  uint32_t* const xt = p.hdr->pad;
+  const int r_t = (int)p.xrows[0];
+  if (blockIdx.x == 0 && tid == 0) xt[0] = (uint32_t)wall_clock64();
+  // per-workgroup totals (ticks of 10 ns): claiming, idle waits, chunk work,
+  // chunk completion, pass finishing; chunks done, failed claims
+  uint32_t a_claim = 0, a_idle = 0, a_work = 0, a_done = 0, a_fin = 0, a_n = 0, a_fail = 0;
+  uint64_t t_mark = wall_clock64();
+  auto lap = [&](uint32_t& acc) {
+    const uint64_t t = wall_clock64();
+    acc += (uint32_t)(t - t_mark);
+    t_mark = t;
+  };
+#define RK_LAP(acc) lap(acc)
+#else
+#define RK_LAP(acc) do {} while (0)
+#endif
   for (;;) {
     // Claim a chunk: wave 0 looks at 64 rows of the list at a time, from
     // this workgroup's cursor, and claims in the first row with chunks left.
@@ -1810,6 +1939,9 @@ void k_exact(KParams p) {
                                          __HIP_MEMORY_SCOPE_AGENT);
           old = __shfl(old, l, 64);
           const int rl = __shfl(r, l, 64);
+#ifdef RK_PHASE_TIMING
+          if (!((uint32_t)(old >> 32) != XSEQ_DONE && (uint32_t)old < p.xst[rl].nch)) ++a_fail;
+#endif
           if ((uint32_t)(old >> 32) != XSEQ_DONE && (uint32_t)old < p.xst[rl].nch) {
             got = rl;
             gc = (uint32_t)old;
@@ -1834,25 +1966,53 @@ void k_exact(KParams p) {
       // workgroups that hold the last chunks of a pass to open the next one.
       const uint32_t left = __hip_atomic_load(&p.hdr->x_left, __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_AGENT);
-      if (left == 0) return;
+      if (left == 0) {
+#ifdef RK_PHASE_TIMING
+        RK_LAP(a_idle);
+        if (blockIdx.x == 0 && tid == 0) xt[1] = (uint32_t)wall_clock64();
+        if (tid == 0) {
+          atomicAdd(&xt[30], a_claim);
+          atomicAdd(&xt[31], a_idle);
+          atomicAdd(&xt[32], a_work);
+          atomicAdd(&xt[33], a_done);
+          atomicAdd(&xt[34], a_fin);
+          atomicAdd(&xt[35], a_n);
+          atomicAdd(&xt[36], a_fail);
+          atomicAdd(&xt[37], 1u);
+        }
+#endif
+        return;
+      }
       const uint64_t now = wall_clock64();
       if (wait0 == 0) wait0 = now;
       if (now - wait0 > X_MAX_WAIT) {
         if (tid == 0) atomicOr(&p.hdr->status, (uint32_t)ST_INTERNAL);
         return;
       }
-      __builtin_amdgcn_s_sleep(8);
+      // back off while idle (up to ~1.7 us between polls), so that waiting
+      // workgroups do not crowd the claim words' memory channel
+      for (uint32_t i = 0; i < idle_naps; ++i) __builtin_amdgcn_s_sleep(2);
+      idle_naps = idle_naps < 32 ? idle_naps * 2 : 32;
+      RK_LAP(a_idle);
       continue;
     }
+    RK_LAP(a_claim);
     wait0 = 0;
+    idle_naps = 1;
+#ifdef RK_PHASE_TIMING
+    if (r == r_t && c == 0 && tid == 0 && seq < 12) xt[4 + 2 * seq] = (uint32_t)wall_clock64();
+#endif
     // The row's pass state, as the workgroup that opened this pass left it
-    // (read after the claim that returned the pass's sequence number).
-    RowState st;
-    {
+    // (read after the claim that returned the pass's sequence number; it
+    // does not change within a pass, so a workgroup keeps it for the next
+    // chunk of the same pass).
+    if (r != st_r || seq != st_seq) {
       const uint32_t* src = reinterpret_cast<const uint32_t*>(p.st + r);
       uint32_t* dst = reinterpret_cast<uint32_t*>(&st);
 #pragma unroll
       for (int i = 0; i < (int)(sizeof(RowState) / 4); ++i) dst[i] = ld_dev(src + i);
+      st_r = r;
+      st_seq = seq;
     }
     const RowDesc d = p.desc[r];  // constant in this launch: plain loads
     const uint32_t nch = p.xst[r].nch, ct = p.xst[r].ct;
@@ -1864,12 +2024,16 @@ void k_exact(KParams p) {
     } else if (st.phase == PH_FILTER) {
       filter_segment<DT, LARGEST, U>(p, lds.f, st, d, p.cand_base[r], r, ta, tb);
     } else if (st.phase == PH_ORDERED && !(st.flags & RF_EMIT_WRITE)) {
-      emit_count_chunk<DT, LARGEST>(p, st, d, r, c, ta, tb, scratch);
+      emit_count_chunk<DT, LARGEST, U>(p, st, d, r, c, ta, tb, scratch);
     } else if (st.phase == PH_ORDERED) {
-      emit_write_chunk<DT, LARGEST>(p, st, d, r, c, ta, tb, scratch);
+      emit_write_chunk<DT, LARGEST, U>(p, st, d, r, c, ta, tb, scratch);
     } else if (tid == 0) {
       atomicOr(&p.hdr->status, (uint32_t)ST_INTERNAL);  // a claimed chunk in no pass
     }
+#ifdef RK_PHASE_TIMING
+    ++a_n;
+#endif
+    RK_LAP(a_work);
     // Chunk done: this workgroup's histogram atomics and chunk counts
     // (device-coherent) are performed before the count that may let another
     // workgroup finish the pass.  (Candidates and outputs are read by later
@@ -1882,8 +2046,16 @@ void k_exact(KParams p) {
       last_sh = prev + 1 == nch ? 1u : 0u;
     }
     __syncthreads();
+    RK_LAP(a_done);
     if (!last_sh) continue;
     // This workgroup completed the pass: the row's next stage.
+#ifdef RK_PHASE_TIMING
+    if (r == r_t && tid == 0 && seq < 12) {
+      xt[5 + 2 * seq] = (uint32_t)wall_clock64();
+      xt[2] = nch;
+      xt[3] = seq + 1;
+    }
+#endif
     bool leave;
     if (st.phase == PH_HIST) {
       leave = finish_hist<KB>(p, r, st, lds.h.sub[0], scratch, outv);
@@ -1910,7 +2082,9 @@ void k_exact(KParams p) {
       }
     }
     __syncthreads();
+    RK_LAP(a_fin);
   }
+#undef RK_LAP
 }
 
 template <int DT, bool LARGEST>
@@ -2057,7 +2231,11 @@ static hipError_t run_typed_u(const KParams& p, const Launch& L0) {
   hipStream_t s = L0.stream;
   // Persistent streaming kernels: every workgroup resident from the start.
   const int fgrid = std::min(L0.grid, resident_grid(k_filter<DT, LARGEST, U>, STREAM_NT));
-  const int xgrid = std::min(L0.grid, resident_grid(k_exact<DT, LARGEST, U>, STREAM_NT));
+  // The exact-pass executor: the call's grid on the exact path, else its
+  // own default (the grid of a sampled call is the filter's).
+  const int xgrid = std::min((p.flags & FL_EXACT) ? L0.grid
+                                                  : device_cus() * exact_grid_per_cu(p.total_len),
+                             resident_grid(k_exact<DT, LARGEST, U>, STREAM_NT));
   // Per-row kernels loop over rows: one resident wave of workgroups suffices.
   auto rows = [&](int resident) { return std::max(1, std::min(L0.row_grid, resident)); };
   const int g_sample = rows(resident_grid(k_sample<DT, LARGEST>, SEL_NT, dyn_sample));

@@ -166,6 +166,12 @@ enum : uint32_t {
   FL_EXACT = 8u,
 };
 
+// Workgroups per CU of the exact-pass executor by default: 1, or 2 for calls
+// of 2^26 keys and more (measured on MI355X: fewer workgroups contend less
+// for the device-scope atomics every chunk pays; more stream faster once the
+// work is large).
+inline int exact_grid_per_cu(int64_t total_len) { return total_len >= (int64_t(1) << 26) ? 2 : 1; }
+
 constexpr int HIST_MAX_BINS = 2048;
 constexpr int TILE_MIN = 1024;
 

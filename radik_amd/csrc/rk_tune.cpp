@@ -89,7 +89,7 @@ int rk_autotune_batched(int dtype, const void* vals, const int64_t* row_off,
   // 3 x 11 or 4 x 8 bits for 32-bit keys) for the exact path.
   std::vector<rk_tune> cands;
   const bool exact = (flags & RK_EXACT) != 0;
-  const std::vector<int> gms = exact ? std::vector<int>{4, 8, 16} : std::vector<int>{2, 3, 4, 8};
+  const std::vector<int> gms = exact ? std::vector<int>{1, 2, 4} : std::vector<int>{2, 3, 4, 8};
   for (int tile : {4096, 16384, 65536})
     for (int gm : gms)
       for (int un : {2, 4}) {
