@@ -227,16 +227,19 @@ int launch(int dtype, const void* vals, const int64_t* row_off, const void* cons
   while (ct > 1 && (int64_t)tiles_bound / ct < 2 * (int64_t)grid) ct >>= 1;
   p.chunk_tiles = (int32_t)ct;
   // The exact-pass executor hands rows out in chunks of about two per
-  // workgroup of the call's whole work, within [16K, 1M] keys: each chunk
-  // pays a few device-scope round trips (claim, histogram merge,
-  // completion), which larger chunks amortise, and a single row still
-  // splits into many.
+  // workgroup of the call's whole work, but of at least 16K keys (or 1/64
+  // of a small call), within [4K, 1M] keys: each chunk pays a few
+  // device-scope round trips (claim, histogram merge, completion), which
+  // larger chunks amortise, and a single row still splits into many
+  // (measured on MI355X: 16K-key chunks beat 4K ones from 2^20 keys, 4K
+  // beat 16K on a 2^17-key row).
   {
     const int64_t xg = (flags & RK_EXACT)
                            ? std::max<int64_t>(1, std::min<int64_t>(grid, (int64_t)di.cus * 4))
                            : (int64_t)di.cus * rk::exact_grid_per_cu(total_len);
-    const int64_t xe = std::min<int64_t>(int64_t(1) << 20,
-                                         std::max<int64_t>(16384, total_len / (2 * xg)));
+    const int64_t want = std::max<int64_t>(total_len / (2 * xg),
+                                           std::min<int64_t>(16384, total_len / 64));
+    const int64_t xe = std::min<int64_t>(int64_t(1) << 20, std::max<int64_t>(4096, want));
     p.x_chunk_tiles = (int32_t)std::max<int64_t>(1, (xe + p.tune.tile - 1) / p.tune.tile);
   }
   const hipError_t e =

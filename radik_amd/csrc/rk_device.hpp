@@ -461,12 +461,13 @@ RK_DEV void load_to_lds(D* dst, const S* src, uint32_t n, F&& f) {
 // Finds, scanning from the top bin down, the bin b with
 //   above(b) < k <= above(b) + hist[b]
 // where above(b) = sum of hist over bins > b.  Result in out[0..2] =
-// {b, above(b), hist[b]}; out[0] = 0xffffffff if sum(hist) < k.
-// Thread t owns a contiguous run of bins counted from the top, so an
-// exclusive prefix scan over threads yields above().
+// {b, above(b), hist[b]}; out[0] = 0xffffffff if sum(hist) < k; sum(hist)
+// in *total_out if given.  Thread t owns a contiguous run of bins counted
+// from the top, so an exclusive prefix scan over threads yields above().
 template <int NT>
 RK_DEV void block_find_crossing(const uint32_t* hist, int nbins, uint32_t k,
-                                uint32_t* scratch, uint32_t* out) {
+                                uint32_t* scratch, uint32_t* out,
+                                uint32_t* total_out = nullptr) {
   if (threadIdx.x == 0) out[0] = 0xffffffffu;
   const int per = (nbins + NT - 1) / NT;
   const int hi = nbins - (int)threadIdx.x * per;
@@ -477,6 +478,7 @@ RK_DEV void block_find_crossing(const uint32_t* hist, int nbins, uint32_t k,
   }
   uint32_t total;
   const uint32_t above = block_excl_scan<NT>(s, scratch, total);  // has barriers
+  if (total_out && threadIdx.x == 0) *total_out = total;
   if (s != 0 && above < k && k <= above + s) {
     uint32_t a = above;
     for (int j = 0; j < per; ++j) {

@@ -510,6 +510,8 @@ RK_DEV void exact_start_row(const KParams& p, int r, uint32_t why) {
     XRow& x = p.xst[r];
     x.claim = 0;
     x.done = 0;
+    x.kor = 0u;
+    x.kand = 0xffffffffu;
     x.ct = ct;
     x.nch = (tiles + ct - 1) / ct;
     const uint32_t slot = atomicAdd(&p.hdr->x_rows, 1u);
@@ -1526,10 +1528,13 @@ RK_DEV void hist_segment(const KParams& p, HistLds& lds, const RowState& st, con
   // One key: counted when it carries the resolved prefix.  Within the lanes
   // that count it, the first one adds for all of them when they share a
   // bin (readfirstlane / ballot, no lane exchange through LDS).
+  uint32_t kor = 0u, kand = 0xffffffffu;  // of the keys this thread counts
   auto count = [&](uint32_t u, bool valid) {
     const uint32_t sk = sel_key<DT, LARGEST>(u);
     const bool ok = valid && (b == 0 || (uint32_t)((uint64_t)sk >> (KB - b)) == P0);
     if (ok) {
+      kor |= sk;
+      kand &= sk;
       const uint32_t dg = (sk >> shift) & (nb - 1);
       const uint32_t d0 = __builtin_amdgcn_readfirstlane(dg);
       const uint64_t act = __ballot(true);
@@ -1575,7 +1580,16 @@ RK_DEV void hist_segment(const KParams& p, HistLds& lds, const RowState& st, con
       for (int u = 0; u < U; ++u) x[u] = y[u];
     }
   }
-  __syncthreads();
+  {
+    // the workgroup's OR and AND into the row's (its barriers also complete
+    // the LDS histograms); OR of nothing is 0, AND of nothing all ones
+    __shared__ uint64_t sor[NW];
+    const uint64_t both = block_or64<STREAM_NT>(((uint64_t)(~kand) << 32) | kor, sor);
+    if (threadIdx.x == 0 && both != 0) {
+      atomicOr(&p.xst[r].kor, (uint32_t)both);
+      atomicAnd(&p.xst[r].kand, ~(uint32_t)(both >> 32));
+    }
+  }
 #ifdef RK_PHASE_TIMING
   const uint64_t hs2 = wall_clock64();
   uint32_t nz = 0;
@@ -1809,18 +1823,34 @@ RK_DEV bool finish_hist(const KParams& p, int r, const RowState& s, uint32_t* hi
     }
   }
   __syncthreads();
-  block_find_crossing<STREAM_NT>(hist, (int)nb, s.rem, scratch, outv);
+  block_find_crossing<STREAM_NT>(hist, (int)nb, s.rem, scratch, outv, &outv[3]);
   if (threadIdx.x == 0) {
     RowState& st = p.st[r];
     const uint32_t bin = outv[0];
+    const uint32_t counted = outv[3];
     uint32_t leave = 0;
     if (bin == 0xffffffffu) {
       atomicOr(&p.hdr->status, (uint32_t)ST_INTERNAL);
       st_dev(&st.phase, (uint32_t)PH_DONE);
       leave = 1;
     } else {
-      const uint32_t prefix = (s.prefix << w) | bin;
-      const uint32_t resolved = (uint32_t)(b + w);
+      uint32_t prefix = (s.prefix << w) | bin;
+      uint32_t resolved = (uint32_t)(b + w);
+      if (outv[2] == counted && resolved < (uint32_t)KB) {
+        // Every key the pass counted is in this bin: they share the bits
+        // down to their common prefix (OR == AND there), which the next
+        // window skips, as RadiK's adaptive scaling spreads skewed rows
+        // (utils.cuh:255-324), but exactly, in key space.
+        const uint32_t kor = ld_dev(&p.xst[r].kor), kand = ld_dev(&p.xst[r].kand);
+        const uint32_t diff = kor ^ kand;
+        const uint32_t cp = diff == 0u ? (uint32_t)KB : (uint32_t)(__clz(diff) - (32 - KB));
+        if (cp > resolved) {
+          resolved = cp;
+          prefix = resolved >= 32u ? kand : (kand >> (KB - (int)resolved));
+        }
+      }
+      st_dev(&p.xst[r].kor, 0u);
+      st_dev(&p.xst[r].kand, 0xffffffffu);
       const uint32_t rem = s.rem - outv[1];
       const uint32_t k = (uint32_t)p.desc[r].k;
       const uint64_t need = (uint64_t)(k - rem) + outv[2];
@@ -1871,10 +1901,11 @@ RK_DEV void finish_count(const KParams& p, int r, uint32_t nch, uint32_t* scratc
 // near it (a chunk's pass plus a bin selection take microseconds).
 constexpr uint64_t X_MAX_WAIT = 100000000ull;  // 1 s
 
-// LDS (a 32 KB union of the histogram and candidate-staging buffers) allows
-// 4 workgroups per CU: registers are bounded to match.
+// The executor runs 1 or 2 workgroups per CU by default (4 at most from the
+// autotuner's candidates): registers are bounded for 2 waves per SIMD, which
+// keeps every variant free of scratch spills.
 template <int DT, bool LARGEST, int U>
-__global__ __launch_bounds__(STREAM_NT) __attribute__((amdgpu_waves_per_eu(U >= 8 ? 1 : 4)))
+__global__ __launch_bounds__(STREAM_NT) __attribute__((amdgpu_waves_per_eu(U >= 8 ? 1 : 2)))
 void k_exact(KParams p) {
   const uint32_t nrows = min(p.hdr->x_rows, (uint32_t)p.R);  // set by earlier launches
   if (nrows == 0) return;

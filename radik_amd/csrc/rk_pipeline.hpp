@@ -70,6 +70,8 @@ struct XRow {
   uint32_t done;   // chunks of the current stage completed
   uint32_t nch;    // chunks per stage
   uint32_t ct;     // tiles per chunk
+  uint32_t kor;    // histogram pass: OR and AND of the keys it counted, so
+  uint32_t kand;   //   that the next window can skip the bits they all share
   uint32_t pad;
 };
 constexpr uint32_t XSEQ_DONE = 0xffffffffu;  // claim's sequence once the row has left
