@@ -8,16 +8,17 @@
 //   plan_rows, plan_scan            row descriptors, tile list, candidate slots
 //   sample                          long rows: threshold T from a sample
 //   filter                          one streaming read: compact keys >= T
-//   select_finalize                 LDS radix select + bitonic sort + write
+//   select_finalize                 per row: sort (or radix select, then
+//                                   sort) of the candidates; write
 //   merge_sort, merge_rank          (max_k > 4096) rows with k > 4096: the
 //                                   sort split over workgroups
-//   [exact_init], (hist, hist_select) x passes
-//                                   exact RadiK passes for rows the sampled
-//                                   path could not settle (and the plain path)
-//   filter, select_finalize         second round for those rows
-//   emit_count, emit_write, finalize_ordered
-//                                   rows whose ties at the k-th key overflow
-//                                   the candidate buffer: ordered emit
+//   [exact_init], exact             the exact RadiK passes for rows the
+//                                   sampled path could not settle (and the
+//                                   plain path) in one persistent executor:
+//                                   digit histograms, then a filter read or
+//                                   the ordered emit's count and write reads
+//   select_finalize                 second round for those rows, and the
+//                                   ordered rows' index mapping / tail fill
 //
 // Selection order everywhere: the 64-bit composite key
 //   (sel_key << 32) | ~row_position
@@ -1072,6 +1073,40 @@ RK_DEV void sort_and_write(const KParams& p, const RowDesc& d, uint64_t* sbuf, u
   write_row<DT, LARGEST>(p, d, sbuf, m, mode);
 }
 
+// Rows of the ordered emit, after the executor wrote them (the last step of
+// the fallback round's select): position order needs only the caller's
+// indices and the tail fill (any k, no LDS); the other orders sort the k
+// keys the emit wrote.  Rows are taken by workgroup index.
+template <int DT, bool LARGEST>
+RK_DEV void finalize_ordered_rows(const KParams& p, uint64_t* sbuf, const SortLds& sl, int mode) {
+  using U = typename Traits<DT>::U;
+  for (int r = blockIdx.x; r < p.R; r += gridDim.x) {
+    if (p.st[r].phase != PH_ORDERED) continue;
+    const RowDesc d = p.desc[r];
+    const uint32_t m = (uint32_t)d.k;
+    if (mode == ORD_POS) {
+      U* ow = static_cast<U*>(d.out_v);
+      if (d.idx_in)
+        for (uint32_t i = threadIdx.x; i < m; i += SEL_NT)
+          put_idx(p, d, i, caller_idx(p, d, (uint32_t)get_idx(p, d, i)));
+      for (int64_t i = (int64_t)m + threadIdx.x; i < d.slots; i += SEL_NT) {
+        ow[i] = (U)0;
+        put_idx(p, d, i, -1);
+      }
+    } else {
+      const U* ov = static_cast<const U*>(d.out_v);
+      for (uint32_t i = threadIdx.x; i < m; i += SEL_NT)
+        sbuf[i] = order_key<DT>(mode, sel_key<DT, LARGEST>((uint32_t)ov[i]),
+                                (uint32_t)get_idx(p, d, i));
+      __syncthreads();
+      sort_and_write<DT, LARGEST>(p, d, sbuf, m, mode, sl);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) p.st[r].phase = PH_DONE;
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // select_finalize: per row, one 1024-thread workgroup with a 16384-key LDS
 // buffer.  COPYALL rows are sorted whole; SMALL rows are loaded straight from
@@ -1084,7 +1119,10 @@ RK_DEV void sort_and_write(const KParams& p, const RowDesc& d, uint64_t* sbuf, u
 // select's loops hold 8 loads in flight and the 4096-key bucket sort 8 keys
 // per thread in registers, which a 96- or 80-register budget spills to
 // scratch memory (gfx950, 96: 72-156 bytes per lane).
-template <int DT, bool LARGEST>
+// FIN: the instance launched after the exact-pass executor, which also
+// finalizes the rows it emitted in order (a separate instance, so that the
+// sampled round's select keeps its own, smaller code).
+template <int DT, bool LARGEST, bool FIN = false>
 __global__ __launch_bounds__(SEL_NT) __attribute__((amdgpu_waves_per_eu(4)))
 void k_select_finalize(KParams p) {
   if (p.round && p.hdr->fallback_rows == 0) return;
@@ -1291,6 +1329,8 @@ void k_select_finalize(KParams p) {
     if (threadIdx.x == 0) p.st[r].phase = next_phase;
     __syncthreads();
   }
+  // the rows the executor emitted in order (none before it has run)
+  if (FIN && p.hdr->ordered_rows != 0) finalize_ordered_rows<DT, LARGEST>(p, sbuf, sl, mode);
   if (p.publish && threadIdx.x == 0) {
     // the last workgroup to finish copies the header (every other
     // workgroup's header updates precede its increment of sel_done)
@@ -1718,14 +1758,15 @@ RK_DEV void emit_write_chunk(const KParams& p, const RowState& st, const RowDesc
     uint32_t qb = run_e + (ex >> 16);
 #pragma unroll
     for (int e = 0; e < NMAX; ++e) {
-      if ((uint32_t)e >= n) break;
-      const uint32_t sk = sel_key<DT, LARGEST>(u[e]);
+      const uint32_t sk = (uint32_t)e < n ? sel_key<DT, LARGEST>(u[e]) : 0u;
       const uint32_t pos = (uint32_t)(first + e);
-      if (sk > V) {
+      if ((uint32_t)e >= n) {
+        // past this thread's elements
+      } else if (sk > V) {
         const uint32_t o = gb + (qb < rem ? qb : rem);
         if (o < kk) {
           ov[o] = (UT)u[e];
-          put_idx(p, d, o, pos);  // a position: mapped by k_finalize_ordered
+          put_idx(p, d, o, pos);  // a position: mapped by finalize_ordered_rows
         }
         ++gb;
       } else if (sk == V) {
@@ -2118,45 +2159,6 @@ void k_exact(KParams p) {
 #undef RK_LAP
 }
 
-template <int DT, bool LARGEST>
-__global__ __launch_bounds__(SEL_NT) void k_finalize_ordered(KParams p) {
-  if (p.hdr->ordered_rows == 0) return;
-  using U = typename Traits<DT>::U;
-  extern __shared__ uint64_t sbuf[];
-  __shared__ uint32_t bins[HIST_MAX_BINS];
-  __shared__ uint64_t s64[3 * SEL_NT / 64];
-  __shared__ uint32_t s32[SEL_NT / 64];
-  __shared__ uint32_t ctab[256];
-  const SortLds sl{bins, s64, s32, ctab, (uint32_t)p.lds_keys};
-  const int mode = order_mode(p.flags);
-  for (int r = blockIdx.x; r < p.R; r += gridDim.x) {
-    if (p.st[r].phase != PH_ORDERED) continue;
-    const RowDesc d = p.desc[r];
-    const uint32_t m = (uint32_t)d.k;
-    if (mode == ORD_POS) {
-      // the emit wrote values and positions in position order already: map
-      // positions through idx_in and write the tail fill (any k, no LDS)
-      U* ow = static_cast<U*>(d.out_v);
-      if (d.idx_in)
-        for (uint32_t i = threadIdx.x; i < m; i += SEL_NT)
-          put_idx(p, d, i, caller_idx(p, d, (uint32_t)get_idx(p, d, i)));
-      for (int64_t i = (int64_t)m + threadIdx.x; i < d.slots; i += SEL_NT) {
-        ow[i] = (U)0;
-        put_idx(p, d, i, -1);
-      }
-    } else {
-      const U* ov = static_cast<const U*>(d.out_v);
-      for (uint32_t i = threadIdx.x; i < m; i += SEL_NT)
-        sbuf[i] = order_key<DT>(mode, sel_key<DT, LARGEST>((uint32_t)ov[i]),
-                                (uint32_t)get_idx(p, d, i));
-      __syncthreads();
-      sort_and_write<DT, LARGEST>(p, d, sbuf, m, mode, sl);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) p.st[r].phase = PH_DONE;
-    __syncthreads();
-  }
-}
 
 // ---------------------------------------------------------------------------
 // Host side: the launch sequence.
@@ -2175,7 +2177,7 @@ static void set_lds_attrs(size_t dyn) {
                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_select_finalize<DT, LARGEST>),
                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_finalize_ordered<DT, LARGEST>),
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_select_finalize<DT, LARGEST, true>),
                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_merge_rank<DT, LARGEST>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
@@ -2272,7 +2274,6 @@ static hipError_t run_typed_u(const KParams& p, const Launch& L0) {
   const int g_sample = rows(resident_grid(k_sample<DT, LARGEST>, SEL_NT, dyn_sample));
   const int g_select = rows(resident_grid(k_select_finalize<DT, LARGEST>, SEL_NT, dyn));
   const int g_init = rows(resident_grid(k_exact_init, 256));
-  int g_ford = rows(resident_grid(k_finalize_ordered<DT, LARGEST>, SEL_NT, dyn));
   KParams p1 = p;  // fallback round: kernels return at once when no row fell back
   p1.round = 1;
   // In a sampled call the fallback round is almost always idle, and an idle
@@ -2284,7 +2285,6 @@ static hipError_t run_typed_u(const KParams& p, const Launch& L0) {
   if (!exact_call) {
     const int cus = device_cus();
     fb_sel = std::min(fb_sel, cus);
-    g_ford = std::min(g_ford, cus);
   }
   if (L0.part != PART_FALLBACK) {
     if (p.R <= 1024) {
@@ -2322,8 +2322,7 @@ static hipError_t run_typed_u(const KParams& p, const Launch& L0) {
   k_exact<DT, LARGEST, U><<<xgrid, STREAM_NT, 0, s>>>(p1);
   // COPYALL / SMALL rows of an exact-path call are finalized here too.
   KParams pf = exact_call ? p : p1;
-  k_select_finalize<DT, LARGEST><<<fb_sel, SEL_NT, dyn, s>>>(pf);
-  k_finalize_ordered<DT, LARGEST><<<g_ford, SEL_NT, dyn, s>>>(p1);
+  k_select_finalize<DT, LARGEST, true><<<fb_sel, SEL_NT, dyn, s>>>(pf);
   return hipGetLastError();
 }
 
