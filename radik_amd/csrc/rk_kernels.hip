@@ -2281,6 +2281,19 @@ static hipError_t run_typed_u(const KParams& p, const Launch& L0) {
   // kernels run one workgroup per CU.  The exact-pass executor keeps its
   // full grid, so that rows that do fall back stream at full bandwidth.
   const bool exact_call = (p.flags & FL_EXACT) != 0;
+  // Rows with k > MERGE_CHUNK: the select compacts their k keys and these
+  // two kernels sort them over many workgroups (4096-key chunks, then every
+  // key placed by its rank), instead of one workgroup's sort of all k.
+  auto launch_merge = [&](const KParams& pm) {
+    const int cpr = merge_chunks(pm);
+    const int64_t items = (int64_t)pm.R * cpr;
+    const size_t drank = (size_t)cpr * MERGE_CHUNK * sizeof(uint64_t);
+    const int g_ms = (int)std::min<int64_t>(items, resident_grid(k_merge_sort<DT>, MS_NT));
+    const int g_mr = (int)std::min<int64_t>(
+        items, resident_grid(k_merge_rank<DT, LARGEST>, SEL_NT, drank));
+    k_merge_sort<DT><<<g_ms, MS_NT, 0, s>>>(pm);
+    k_merge_rank<DT, LARGEST><<<g_mr, SEL_NT, drank, s>>>(pm);
+  };
   int fb_sel = g_select;
   if (!exact_call) {
     const int cus = device_cus();
@@ -2304,25 +2317,22 @@ static hipError_t run_typed_u(const KParams& p, const Launch& L0) {
     pm.publish = L0.part == PART_MAIN ? L0.publish : nullptr;
     pm.merge = merge_chunks(p) > 1;
     k_select_finalize<DT, LARGEST><<<g_select, SEL_NT, dyn, s>>>(pm);
-    if (pm.merge) {
-      const int cpr = merge_chunks(p);
-      const int64_t items = (int64_t)p.R * cpr;
-      const size_t drank = (size_t)cpr * MERGE_CHUNK * sizeof(uint64_t);
-      const int g_ms = (int)std::min<int64_t>(items, resident_grid(k_merge_sort<DT>, MS_NT));
-      const int g_mr = (int)std::min<int64_t>(
-          items, resident_grid(k_merge_rank<DT, LARGEST>, SEL_NT, drank));
-      k_merge_sort<DT><<<g_ms, MS_NT, 0, s>>>(pm);
-      k_merge_rank<DT, LARGEST><<<g_mr, SEL_NT, drank, s>>>(pm);
-    }
+    if (pm.merge) launch_merge(pm);
   }
   if (L0.part == PART_MAIN) return hipGetLastError();
   // rows entering the exact passes are initialised by the kernel that sends
   // them there; only an exact call starts them all here
   if (exact_call) k_exact_init<<<g_init, 256, 0, s>>>(p1);
   k_exact<DT, LARGEST, U><<<xgrid, STREAM_NT, 0, s>>>(p1);
-  // COPYALL / SMALL rows of an exact-path call are finalized here too.
+  // COPYALL / SMALL rows of an exact-path call are finalized here too.  An
+  // exact call's large selections are sorted by the merge kernels, as in a
+  // sampled call's first round (a sampled call's rare fallback rows with
+  // k > MERGE_CHUNK sort in their select's workgroup, so that its fallback
+  // round stays two launches).
   KParams pf = exact_call ? p : p1;
+  pf.merge = exact_call && merge_chunks(p) > 1;
   k_select_finalize<DT, LARGEST, true><<<fb_sel, SEL_NT, dyn, s>>>(pf);
+  if (pf.merge) launch_merge(pf);
   return hipGetLastError();
 }
 

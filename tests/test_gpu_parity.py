@@ -402,10 +402,11 @@ def test_workload_c1_full_size(flags, exact):
 @pytest.mark.parametrize("dt", ["f32", "bf16", "f16"])
 @pytest.mark.parametrize("flags", FLAG_SETS)
 def test_large_k_chunked_sort(dt, flags):
-    """Sampled rows with k above one sort chunk (4096 keys) are sorted in
-    chunks by several workgroups and ranked across chunks: k at and around
-    the chunk multiples, duplicate-heavy rows (ties broken by position),
-    caller indices, mixed with small-k and LDS-direct rows in one call."""
+    """Rows with k above one sort chunk (4096 keys) are sorted in chunks by
+    several workgroups and ranked across chunks, on the sampled path and on
+    the exact path: k at and around the chunk multiples, duplicate-heavy rows
+    (ties broken by position), caller indices, mixed with small-k and
+    LDS-direct rows in one call."""
     rng = np.random.default_rng(301 + DT_CODE[dt] * 7 + flags)
     lens = np.array([200_000, 300_001, 150_000, 1 << 20, 90_000, 600_000, 12_000, 70_000],
                     np.int64)
@@ -420,8 +421,9 @@ def test_large_k_chunked_sort(dt, flags):
         pos = int(off[r]) + rng.choice(int(lens[r]), cnt, replace=False)
         bits[pos] = _bits_of(dt, np.full(cnt, v, np.float32))
     idx_in = rng.integers(-2**31, 2**31 - 1, int(off[-1])).astype(np.int32)
-    _check(dt, bits, off, ks, flags)
-    _check(dt, bits, off, ks, flags, idx_in=idx_in)
+    for exact in (0, EXACT):
+        _check(dt, bits, off, ks, flags | exact)
+        _check(dt, bits, off, ks, flags | exact, idx_in=idx_in)
 
 
 @pytest.mark.parametrize("dt", ["f32", "bf16", "f16"])
