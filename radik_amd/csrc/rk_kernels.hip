@@ -1958,7 +1958,7 @@ void k_exact(KParams p) {
   __shared__ uint32_t scratch[16];
   __shared__ uint32_t outv[4];
   __shared__ int r_sh;
-  __shared__ uint32_t c_sh, seq_sh, last_sh;
+  __shared__ uint32_t c_sh, seq_sh, last_sh, cont_sh;
   const int tid = threadIdx.x;
   uint32_t cursor = blockIdx.x % nrows;
   uint64_t wait0 = 0;
@@ -1985,10 +1985,11 @@ void k_exact(KParams p) {
 #else
 #define RK_LAP(acc) do {} while (0)
 #endif
+  bool cont = false;  // this workgroup opened a pass and took its first chunk
   for (;;) {
     // Claim a chunk: wave 0 looks at 64 rows of the list at a time, from
     // this workgroup's cursor, and claims in the first row with chunks left.
-    if (tid < 64) {
+    if (!cont && tid < 64) {
       int got = -1;
       uint32_t gc = 0, gseq = 0;
       for (uint32_t base = 0; base < nrows && got < 0; base += 64) {
@@ -2119,6 +2120,7 @@ void k_exact(KParams p) {
     }
     __syncthreads();
     RK_LAP(a_done);
+    cont = false;
     if (!last_sh) continue;
     // This workgroup completed the pass: the row's next stage.
 #ifdef RK_PHASE_TIMING
@@ -2148,12 +2150,18 @@ void k_exact(KParams p) {
         __hip_atomic_fetch_add(&p.hdr->x_left, 0xffffffffu, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
       } else {
+        // the next pass opens with its first chunk taken by this workgroup,
+        // which goes on to it without a claim round trip
         st_dev(&p.xst[r].done, 0u);
         wait_vmem();
-        st_dev(&p.xst[r].claim, (uint64_t)(seq + 1) << 32);
+        st_dev(&p.xst[r].claim, ((uint64_t)(seq + 1) << 32) | (uint64_t)1);
+        c_sh = 0u;
+        seq_sh = seq + 1;
       }
+      cont_sh = leave ? 0u : 1u;
     }
     __syncthreads();
+    cont = cont_sh != 0;
     RK_LAP(a_fin);
   }
 #undef RK_LAP
