@@ -133,6 +133,34 @@ def test_near_constant_with_duplicates(largest):
             _check("f32", bits, off, ks, flags | (EXACT if exact else 0))
 
 
+@pytest.mark.parametrize("dt", ["f32", "bf16", "f16"])
+def test_exact_passes_skip_shared_key_bits(dt):
+    """The exact passes skip the bits all counted keys share once one bin
+    holds them all: rows whose keys agree on their top 12..31 bits (a few
+    low bits vary, with ties), a constant row (every bit shared: the k-th
+    key is resolved by the skip alone), a row whose values differ only in
+    the last bit, and a negative near-constant row, in both directions and
+    every order, on the exact path and (for comparison) the sampled path."""
+    rng = np.random.default_rng(77 + DT_CODE[dt])
+    dtype = np.uint32 if dt == "f32" else np.uint16
+    rows = []
+    base = {"f32": 0x3F80_0000, "bf16": 0x3F80, "f16": 0x3C00}[dt]  # 1.0
+    nbase = {"f32": 0xBF80_0000, "bf16": 0xBF80, "f16": 0xBC00}[dt]  # -1.0
+    for vary in ((2, 5, 9, 13) if dt == "f32" else (2, 4, 7)):
+        rows.append((base + rng.integers(0, 1 << vary, 40_000)).astype(dtype))
+    rows.append(np.full(30_000, base, dtype))                         # constant
+    rows.append((base + rng.integers(0, 2, 50_000)).astype(dtype))    # last bit only
+    rows.append((nbase + rng.integers(0, 1 << 6, 45_000)).astype(dtype))
+    lens = np.array([len(r) for r in rows], np.int64)
+    off = np.zeros(len(rows) + 1, np.int64)
+    off[1:] = np.cumsum(lens)
+    bits = np.concatenate(rows)
+    ks = np.array([100, 1000, 3000, 17, 500, 2000, 8000][:len(rows)], np.int32)
+    for flags in (LARGEST | SORTED, SORTED | ASCEND, LARGEST, 0, LARGEST | SORTED | ASCEND):
+        for exact in (EXACT, 0):
+            _check(dt, bits, off, ks, flags | exact)
+
+
 @pytest.mark.parametrize("largest", [True, False])
 def test_zipf_rows(largest):
     n = 1 << 19
