@@ -75,7 +75,10 @@ Layout make_layout(int32_t R, int64_t total_len, int32_t max_k) {
   L.tile_cap = E / rk::TILE_MIN + Ru + 1;
   // rows with k beyond the LDS select take no candidate slots
   const uint64_t kc = (uint64_t)std::min(max_k, rk::lds_keys_max());
-  const uint64_t cand_bound = 4 * Ru * kc + E / 64 + 1025 * Ru;
+  // rows with k beyond it (unsorted) bracket their k-th key: len/16 + 4096
+  // slots each
+  const uint64_t big = (uint64_t)max_k > kc ? E / 16 + 4096 * Ru : 0;
+  const uint64_t cand_bound = 4 * Ru * kc + E / 64 + 1025 * Ru + big;
   L.cand_cap = std::min<uint64_t>(E, cand_bound);
   size_t o = 0;
   L.hdr = o; o = align256(o + sizeof(rk::Header));

@@ -506,15 +506,17 @@ RK_DEV void block_find_crossing(const uint32_t* hist, int nbins, uint32_t k,
 // common prefix of all keys (the key-space form of RadiK's adaptive
 // scaling): bits every key shares are never histogrammed.  This form takes
 // the bits that vary (diff) and one key (ref) from its caller.  Loads are
-// issued UN at a time before their LDS atomics.
-template <int NT, int HB, typename KeyT, bool GLOBAL>
+// issued UN at a time before their LDS atomics.  COH: global keys written
+// in the same launch by other workgroups, read with device-coherent loads.
+template <int NT, int HB, typename KeyT, bool GLOBAL, bool COH = false>
 RK_DEV KeyT block_radix_select_from(const KeyT* keys, uint32_t n, uint32_t k, uint64_t diff,
                                     KeyT ref, uint32_t* hist, uint32_t* scratch,
                                     uint32_t* out) {
   constexpr int BITS = (int)sizeof(KeyT) * 8;
   constexpr int UN = 4;
   auto ld = [&](uint32_t i) -> KeyT {
-    if constexpr (GLOBAL) return ldg(keys + i);
+    if constexpr (GLOBAL && COH) return ld_dev(keys + i);
+    else if constexpr (GLOBAL) return ldg(keys + i);
     else return ld_lds(keys + i);
   };
   if (diff == 0) {  // all keys equal: every key is >= ref
@@ -566,12 +568,13 @@ RK_DEV KeyT block_radix_select_from(const KeyT* keys, uint32_t n, uint32_t k, ui
   return prefix;
 }
 
-template <int NT, int HB, typename KeyT, bool GLOBAL>
+template <int NT, int HB, typename KeyT, bool GLOBAL, bool COH = false>
 RK_DEV KeyT block_radix_select(const KeyT* keys, uint32_t n, uint32_t k, uint32_t* hist,
                                uint32_t* scratch, uint64_t* scratch64, uint32_t* out) {
   constexpr int UN = 4;
   auto ld = [&](uint32_t i) -> KeyT {
-    if constexpr (GLOBAL) return ldg(keys + i);
+    if constexpr (GLOBAL && COH) return ld_dev(keys + i);
+    else if constexpr (GLOBAL) return ldg(keys + i);
     else return ld_lds(keys + i);
   };
   const KeyT k0 = ld(0);
@@ -587,7 +590,8 @@ RK_DEV KeyT block_radix_select(const KeyT* keys, uint32_t n, uint32_t k, uint32_
     for (int u = 0; u < UN; ++u) diff |= (uint64_t)(x[u] ^ k0);
   }
   diff = block_or64<NT>(diff, scratch64);
-  return block_radix_select_from<NT, HB, KeyT, GLOBAL>(keys, n, k, diff, k0, hist, scratch, out);
+  return block_radix_select_from<NT, HB, KeyT, GLOBAL, COH>(keys, n, k, diff, k0, hist, scratch,
+                                                            out);
 }
 
 // Descending bitonic sort of P2 (power of two) 64-bit keys held in LDS.

@@ -307,17 +307,19 @@ RK_DEV RowPlan plan_row(const KParams& p, int r) {
     s.phase = PH_SMALL;
   } else {
     // k beyond the LDS select (unsorted output only; the host rejects sorted
-    // calls with such k): exact passes, then the ordered emit, which needs
-    // no candidate slots and writes position order directly.
+    // calls with such k): the ordered emit writes position order directly,
+    // after a sampled bracket of the k-th key (slots for the keys inside
+    // it) or, on the exact path, the digit passes (no slots).
     const bool bigk = k > p.lds_keys;
-    s.phase = (bigk || (p.flags & FL_EXACT)) ? PH_HIST : PH_SAMPLE;
+    s.phase = (p.flags & FL_EXACT) ? PH_HIST : PH_SAMPLE;
     if (s.phase == PH_HIST) s.flags = RF_NEED_INIT;
     tiles = (uint32_t)((len + p.tune.tile - 1) / p.tune.tile);
-    const int64_t cap = bigk ? 0 : 4 * k + (len + 63) / 64 + 1024;
+    const int64_t cap = !bigk ? 4 * k + (len + 63) / 64 + 1024
+                              : ((p.flags & FL_EXACT) ? 0 : len / 16 + 4096);
     s.cap = (uint32_t)(cap < len ? cap : len);
     // A row whose slots hold all of it gains nothing from a threshold: every
     // key becomes a candidate (T = 0) and the select works on the whole row.
-    if (s.phase == PH_SAMPLE && cap >= len) {
+    if (s.phase == PH_SAMPLE && !bigk && cap >= len) {
       s.phase = PH_FILTER;
       s.T = 0;
     }
@@ -490,7 +492,7 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
 // the kernel that makes the decision (after every thread has read the row's
 // state), so the fallback round needs no initialisation launch of its own.
 template <int NT>
-RK_DEV void exact_start_row(const KParams& p, int r, uint32_t why) {
+RK_DEV void exact_start_row(const KParams& p, int r, uint32_t why, uint32_t phase = PH_HIST) {
   __syncthreads();  // the row's state is read before it changes
   uint32_t* h = p.ghist + (size_t)r * HIST_MAX_BINS;
   for (int i = threadIdx.x; i < HIST_MAX_BINS; i += NT) h[i] = 0;
@@ -501,7 +503,7 @@ RK_DEV void exact_start_row(const KParams& p, int r, uint32_t why) {
     w.rem = (uint32_t)p.desc[r].k;
     w.flags = 0;
     w.why = why;
-    w.phase = PH_HIST;
+    w.phase = phase;
     p.cand_cnt[r] = 0;
     // chunks of x_chunk_tiles tiles (sized by the host from the call's work)
     const uint32_t tiles = p.tile_start[r + 1] - p.tile_start[r];
@@ -564,7 +566,9 @@ __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
     const int64_t ge = p.tune.group_elems;  // a multiple of 8, so of PACK
     int64_t ng = p.tune.sample_groups;
     const int64_t ng_cap = p.sample_keys / ge;      // runs that fit the LDS keys
-    if (ng * ge * 64 > len) ng = len / (64 * ge);  // read about 1/64 of the row,
+    // read about 1/64 of the row (a bracketed row, k above the LDS select,
+    // takes the full sample: its bracket narrows with the sample),
+    if (d.k <= p.lds_keys && ng * ge * 64 > len) ng = len / (64 * ge);
     const int64_t ng_min = len / (2 * ge) < 64 ? len / (2 * ge) : 64;
     if (ng < ng_min) ng = ng_min;                   // but at least 64 runs,
     if (ng > ng_cap) ng = ng_cap;                   // and never beyond the LDS
@@ -612,6 +616,39 @@ __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
       kor = (uint32_t)both;
     }  // its barriers also complete the keys
     RK_STAMP(p, r, 1);
+    if (d.k > p.lds_keys) {
+      // k above the LDS select: bracket the k-th key between the sampled
+      // keys of ranks mu -+ (4 sigma + 4) (mu its expected rank, sigma the
+      // binomial spread), so that it lies in [T_lo, T_hi] unless the sample
+      // is unrepresentative, which the bracket pass detects exactly.
+      const float mu = (float)s * (float)d.k / (float)len;
+      const float dl = 4.0f * sqrtf(fmaxf(mu * (1.0f - mu / (float)s), 0.0f)) + 4.0f;
+      const float fh = floorf(mu - dl), fl = ceilf(mu + dl);
+      const uint32_t r_hi = fh < 1.0f ? 1u : (fh > (float)s ? s : (uint32_t)fh);
+      const uint32_t r_lo = fl < 1.0f ? 1u : (fl > (float)s ? s : (uint32_t)fl);
+      const uint32_t t_hi = block_radix_select_from<SEL_NT, 11, uint32_t, false>(
+          keys, s, r_hi, kor ^ kand, kand, hist, scratch, outv);
+      const uint32_t ge_hi = outv[3];
+      __syncthreads();
+      const uint32_t t_lo = block_radix_select_from<SEL_NT, 11, uint32_t, false>(
+          keys, s, r_lo, kor ^ kand, kand, hist, scratch, outv);
+      const uint32_t ge_lo = outv[3];
+      // sampled keys in [T_lo, T_hi] (at least those of equal keys), scaled
+      const double est = (double)(ge_lo - ge_hi + 1u) * (double)len / (double)s;
+      const bool fits = est * 1.25 <= (double)st0.cap;
+      exact_start_row<SEL_NT>(p, r, fits ? FB_NONE : FB_SAMPLE_TOO_MANY,
+                              fits ? (uint32_t)PH_BRACKET : (uint32_t)PH_HIST);
+      if (threadIdx.x == 0) {
+        if (fits) {
+          RowState& w = p.st[r];
+          w.T = t_lo;
+          w.V = t_hi;
+        }
+        atomicAdd(&p.hdr->fallback_rows, 1u);  // the fallback round finishes it
+      }
+      __syncthreads();
+      continue;
+    }
     const float mu = (float)s * (float)d.k / (float)len;
     float rf = ceilf(mu + 4.0f * sqrtf(mu) + 2.0f);
     uint32_t rank = rf >= (float)s ? s : (uint32_t)rf;
@@ -668,6 +705,7 @@ struct AppendCtx {
   uint64_t* cand;      // global candidate slots of the row
   uint32_t* cand_cnt;  // global candidate counter of the row
   uint32_t cap;        // candidate slots of the row
+  bool coh = false;    // device-coherent stores (read back in the same launch)
 };
 
 // Moves the wave's staged keys to the row's candidate slots.  `full` is set
@@ -679,8 +717,11 @@ RK_DEV void wave_flush(const AppendCtx& ac, uint32_t& n, bool& full) {
   uint32_t gb = 0;
   if (lane_id() == 0) gb = atomicAdd(ac.cand_cnt, n);
   gb = __shfl(gb, 0, 64);
-  for (uint32_t i = lane_id(); i < n; i += 64)
-    if (gb + i < ac.cap) ac.cand[gb + i] = ac.stage[i];
+  for (uint32_t i = lane_id(); i < n; i += 64) {
+    if (gb + i >= ac.cap) continue;
+    if (ac.coh) st_dev(ac.cand + gb + i, ac.stage[i]);
+    else ac.cand[gb + i] = ac.stage[i];
+  }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   if (gb + n > ac.cap) full = true;
   n = 0;
@@ -1835,6 +1876,180 @@ RK_DEV void emit_write_chunk(const KParams& p, const RowState& st, const RowDesc
   }
 }
 
+// The bracket pass over tiles [ta, tb) of a PH_BRACKET row (bracket [T, V]
+// in selection keys): the keys above V counted (stored per chunk, the
+// ordered emit's counts start from them) and the keys in [T, V] appended as
+// composite keys, with device-coherent stores: the pass's finisher reads
+// them in this launch.  Called by a whole workgroup.
+template <int DT, bool LARGEST, int U>
+RK_DEV void bracket_segment(const KParams& p, FilterLds& lds, const RowState& st,
+                            const RowDesc& d, int r, uint32_t c, uint32_t ta, uint32_t tb,
+                            uint32_t* scratch) {
+  using UT = typename Traits<DT>::U;
+  constexpr int P = Traits<DT>::PACK;
+  const uint32_t lo = st.T, hi = st.V;
+  const UT* rb = static_cast<const UT*>(d.base);
+  const SegGrid g = seg_grid<DT>(d.base, d.len, p.tune.tile, ta, tb);
+  const int lane = lane_id();
+  const int wave_base = __builtin_amdgcn_readfirstlane(threadIdx.x & ~63);
+  AppendCtx ac;
+  ac.stage = lds.stage[wave_base >> 6];
+  ac.cand = p.cand + p.cand_base[r];
+  ac.cand_cnt = &p.cand_cnt[r];
+  ac.cap = st.cap;
+  ac.coh = true;
+  uint32_t staged = 0;
+  bool full = false;
+  uint32_t gt = 0;
+  // the keys of one 16-byte vector (or one element), counted and appended
+  auto take = [&](const uint4& x, uint32_t valid_mask, int64_t first) {
+    const Vec<DT> v = unpack<DT>(x);
+    uint32_t m = 0;
+#pragma unroll
+    for (int e = 0; e < P; ++e) {
+      const uint32_t sk = sel_key<DT, LARGEST>(v.e[e]);
+      const bool ok = (valid_mask >> e) & 1u;
+      gt += (ok && sk > hi) ? 1u : 0u;
+      m |= (ok && sk >= lo && sk <= hi ? 1u : 0u) << e;
+    }
+    if (__ballot(m != 0u)) append_vec<DT, LARGEST>(ac, staged, full, x, m, first);
+  };
+  if ((wave_base == 0 && g.first_seg) || (wave_base == STREAM_NT - 64 && g.last_seg)) {
+    int64_t pos = -1;
+    if (wave_base == 0 && g.first_seg && lane < g.a0) pos = lane;
+    if (wave_base != 0 && lane < P && g.tail0 + lane < d.len) pos = g.tail0 + lane;
+    uint4 x = make_uint4(0u, 0u, 0u, 0u);
+    if (pos >= 0) x.x = (uint32_t)rb[pos];  // element 0 of the vector
+    take(x, pos >= 0 ? 1u : 0u, pos);
+  }
+  const uint4* vp = reinterpret_cast<const uint4*>(rb + g.al);
+  constexpr uint32_t STEP = (uint32_t)STREAM_NT * U;
+  constexpr int64_t CHUNK = int64_t(1) << 26;
+  for (int64_t c0 = 0; c0 < g.nv; c0 += CHUNK) {
+    const uint32_t nvc = (uint32_t)((g.nv - c0) < CHUNK ? (g.nv - c0) : CHUNK);
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(vp + c0, nvc * 16u);
+    auto load_set = [&](uint4 (&x)[U], uint32_t jb) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) x[u] = bload16(rs, (jb + lane + u * STREAM_NT) * 16u);
+    };
+    uint4 x[U], y[U];
+    uint32_t jb = (uint32_t)wave_base;
+    if (jb < nvc) load_set(x, jb);
+    for (; jb < nvc; jb += STEP) {
+      load_set(y, jb + STEP);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t j = jb + lane + u * STREAM_NT;
+        take(x[u], j < nvc ? (1u << P) - 1u : 0u, g.al + (int64_t)(c0 + j) * P);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) x[u] = y[u];
+    }
+  }
+  // staged keys: one reservation per workgroup, then coherent stores
+  if (lane == 0) lds.flush_n[wave_base >> 6] = staged;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (int w = 0; w < STREAM_NT / 64; ++w) tot += lds.flush_n[w];
+    uint32_t b = tot ? atomicAdd(ac.cand_cnt, tot) : 0u;
+    for (int w = 0; w < STREAM_NT / 64; ++w) {
+      lds.flush_base[w] = b;
+      b += lds.flush_n[w];
+    }
+  }
+  __syncthreads();
+  const uint32_t fb = lds.flush_base[wave_base >> 6];
+  for (uint32_t i = lane; i < staged; i += 64)
+    if (fb + i < ac.cap) st_dev(ac.cand + fb + i, ac.stage[i]);
+  gt = block_sum<STREAM_NT>(gt, scratch);
+  if (threadIdx.x == 0)
+    st_dev(reinterpret_cast<uint64_t*>(p.tile_cnt + p.tile_start[r] + c), (uint64_t)gt);
+}
+
+// The bracket pass of row r is complete.  With G keys above the bracket and
+// n inside it: if G < k <= G + n (and the bracket fit its slots), the k-th
+// key V is the (k - G)-th largest composite key inside (a radix select over
+// them, in global memory), each chunk's keys above V and equal to V are its
+// count above the bracket plus its bracket keys above V, and its bracket keys
+// equal to V (counted by row position in LDS), and the row goes on to the
+// ordered emit's writing pass with their prefix sums.  Otherwise it takes
+// the digit passes from the start.  Returns whether the row leaves.
+template <int DT>
+RK_DEV bool finish_bracket(const KParams& p, int r, const RowState& s, uint32_t nch, uint32_t ct,
+                           HistLds& lh, uint32_t* scratch, uint64_t* scratch64, uint32_t* outv) {
+  constexpr int KB = Traits<DT>::KEY_BITS;
+  const RowDesc d = p.desc[r];
+  const uint32_t k = (uint32_t)d.k;
+  const uint32_t n = ld_dev(p.cand_cnt + r);
+  uint64_t* tc = reinterpret_cast<uint64_t*>(p.tile_cnt + p.tile_start[r]);
+  uint32_t G = 0;
+  for (uint32_t c = threadIdx.x; c < nch; c += STREAM_NT) G += (uint32_t)ld_dev(tc + c);
+  G = block_sum<STREAM_NT>(G, scratch);
+  uint32_t* lgt = &lh.sub[0][0];  // per chunk: bracket keys > V, then == V
+  uint32_t* leq = lgt + 4096;
+  const bool ok = n <= s.cap && G < k && k <= G + n && nch <= 4096u;  // uniform
+  if (!ok) {
+    if (threadIdx.x == 0) {
+      // the digit passes from the start (histogram of the row is still zero)
+      RowState& w = p.st[r];
+      st_dev(&w.prefix, 0u);
+      st_dev(&w.resolved, 0u);
+      st_dev(&w.rem, k);
+      st_dev(&w.why, (uint32_t)FB_BRACKET);
+      st_dev(&w.cap, 0u);
+      st_dev(&w.phase, (uint32_t)PH_HIST);
+    }
+    return false;
+  }
+  const uint64_t* cand = p.cand + p.cand_base[r];
+  const uint64_t kth = block_radix_select<STREAM_NT, 11, uint64_t, true, true>(
+      cand, n, k - G, lh.sub[1], scratch, scratch64, outv);
+  const uint32_t V = (uint32_t)(kth >> 32);
+  for (uint32_t c = threadIdx.x; c < nch; c += STREAM_NT) {
+    lgt[c] = 0u;
+    leq[c] = 0u;
+  }
+  __syncthreads();
+  // the row's vector grid: chunk c holds elements [a0 + c*ct*tile, ...),
+  // the head elements (before a0) are in chunk 0, the tail in the last
+  const SegGrid g = seg_grid<DT>(d.base, d.len, p.tune.tile, 0u, 1u);
+  const int64_t span = (int64_t)ct * p.tune.tile;
+  for (uint32_t i = threadIdx.x; i < n; i += STREAM_NT) {
+    const uint64_t x = ld_dev(cand + i);
+    const uint32_t key = (uint32_t)(x >> 32);
+    if (key < V) continue;
+    const int64_t pos = (int64_t)(~(uint32_t)x);
+    int64_t c = pos < g.a0 ? 0 : (pos - g.a0) / span;
+    if (c > (int64_t)nch - 1) c = (int64_t)nch - 1;
+    atomicAdd(key > V ? &lgt[c] : &leq[c], 1u);
+  }
+  __syncthreads();
+  // chunk prefix sums of (keys > V, keys == V), as the counting pass leaves them
+  uint32_t carry_g = 0, carry_e = 0;
+  for (uint32_t b0 = 0; b0 < nch; b0 += STREAM_NT) {
+    const uint32_t c = b0 + threadIdx.x;
+    const uint32_t gv = c < nch ? (uint32_t)ld_dev(tc + c) + lgt[c] : 0u;
+    const uint32_t ev = c < nch ? leq[c] : 0u;
+    uint32_t tg, te;
+    const uint32_t eg = block_excl_scan<STREAM_NT>(gv, scratch, tg);
+    const uint32_t ee = block_excl_scan<STREAM_NT>(ev, scratch, te);
+    if (c < nch) st_dev(tc + c, ((uint64_t)(carry_e + ee) << 32) | (carry_g + eg));
+    carry_g += tg;
+    carry_e += te;
+  }
+  if (threadIdx.x == 0) {
+    RowState& w = p.st[r];
+    st_dev(&w.V, V);
+    st_dev(&w.rem, k - carry_g);  // keys == V to take (carry_g: all keys > V)
+    st_dev(&w.flags, (uint32_t)RF_EMIT_WRITE);
+    st_dev(&w.phase, (uint32_t)PH_ORDERED);
+    atomicAdd(&p.hdr->ordered_rows, 1u);
+  }
+  (void)KB;
+  return false;
+}
+
 // A histogram pass of row r is complete: the bin of the k-th key (the
 // global histogram is cleared as it is read, for the next pass), then the
 // row's next stage.  Returns (to every thread) whether the row leaves the
@@ -1956,6 +2171,7 @@ void k_exact(KParams p) {
     HistLds h;
   } lds;
   __shared__ uint32_t scratch[16];
+  __shared__ uint64_t scratch64[3 * STREAM_NT / 64];
   __shared__ uint32_t outv[4];
   __shared__ int r_sh;
   __shared__ uint32_t c_sh, seq_sh, last_sh, cont_sh;
@@ -2094,6 +2310,8 @@ void k_exact(KParams p) {
     const uint32_t tb = tiles - ta < ct ? tiles : ta + ct;
     if (st.phase == PH_HIST) {
       hist_segment<DT, LARGEST, U>(p, lds.h, st, d, r, ta, tb);
+    } else if (st.phase == PH_BRACKET) {
+      bracket_segment<DT, LARGEST, U>(p, lds.f, st, d, r, c, ta, tb, scratch);
     } else if (st.phase == PH_FILTER) {
       filter_segment<DT, LARGEST, U>(p, lds.f, st, d, p.cand_base[r], r, ta, tb);
     } else if (st.phase == PH_ORDERED && !(st.flags & RF_EMIT_WRITE)) {
@@ -2133,6 +2351,8 @@ void k_exact(KParams p) {
     bool leave;
     if (st.phase == PH_HIST) {
       leave = finish_hist<KB>(p, r, st, lds.h.sub[0], scratch, outv);
+    } else if (st.phase == PH_BRACKET) {
+      leave = finish_bracket<DT>(p, r, st, nch, ct, lds.h, scratch, scratch64, outv);
     } else if (st.phase == PH_ORDERED && !(st.flags & RF_EMIT_WRITE)) {
       finish_count(p, r, nch, scratch);
       leave = false;

@@ -20,6 +20,10 @@ enum Phase : uint32_t {
                    // buffer: ordered emit in row-position order
   PH_MERGE = 7,    // k > MERGE_CHUNK selected keys (order-keyed) in the row's
                    // candidate slots: sorted in chunks, then ranked and written
+  PH_BRACKET = 8,  // k above the LDS select (unsorted): the sample brackets the
+                   // k-th key in [T, V]; one read counts the keys above V per
+                   // chunk and compacts those in [T, V], the k-th key is
+                   // selected among them, then the ordered emit writes
 };
 
 // Keys per chunk of the multi-workgroup sort of large selections.
@@ -82,6 +86,8 @@ enum Fallback : uint32_t {
   FB_SAMPLE_TOO_MANY = 2,    // estimated candidates exceed the row's slots (heavy ties)
   FB_OVERFLOW = 3,           // more candidates than slots arrived
   FB_TOO_FEW = 4,            // fewer than k candidates were >= T (unrepresentative sample)
+  FB_BRACKET = 5,            // the k-th key fell outside its sampled bracket (or the
+                             // bracket overflowed its slots): digit passes instead
 };
 
 struct Header {
