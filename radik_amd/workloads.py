@@ -137,19 +137,20 @@ def c4(dtype: str = "f32", scale: int = 1, rows: int = 2048) -> Workload:
 
 
 def sample_positions(n: int, sample_groups: int = 1024, group_elems: int = 16, pack: int = 4,
-                     a0: int = 0) -> np.ndarray:
+                     a0: int = 0, full: bool = False) -> np.ndarray:
     """Row positions the sampled path reads for a row of n elements: runs of
     group_elems keys read as 16-byte vectors (pack elements each), run g
     starting at a0 + floor(g * (n - a0 - group_elems) / (ng - 1)) rounded down
     to a multiple of pack, with a0 the row's first element on a 16-byte
     boundary and ng the number of runs the library derives from sample_groups
-    and n (at most 16384 keys, about 1/64 of the row, at least 64 runs).
+    and n (at most 16384 keys, about 1/64 of the row unless `full` (rows with
+    k above the LDS select take the full sample), at least 64 runs).
     Used to build inputs that defeat the sample; the sampled path itself is
     exact whatever the sample sees."""
     ge = group_elems
     keys = min(16384, max(sample_groups, 64) * ge)
     ng = sample_groups
-    if ng * ge * 64 > n:
+    if not full and ng * ge * 64 > n:
         ng = n // (64 * ge)
     ng = max(ng, min(n // (2 * ge), 64))
     ng = max(min(ng, keys // ge), 1)

@@ -1,8 +1,10 @@
 """Unsorted output with k above the LDS sort limit (rk_max_k()): medians and
 other quantile-sized selections (k up to len / 2 and beyond) through the
-exact digit passes and the ordered emit, bit-exact against the oracle's
-position-order output, for every dtype, both directions, ties, idx_in, extra
-tail slots and rows copied whole; sorted calls with such k are refused."""
+sampled bracket of the k-th key (or, on the exact path and when the bracket
+misses, the digit passes) and the ordered emit, bit-exact against the
+oracle's position-order output, for every dtype, both directions, ties,
+idx_in, extra tail slots and rows copied whole; sorted calls with such k are
+refused."""
 
 from __future__ import annotations
 
@@ -114,3 +116,32 @@ def test_sorted_large_k_refused():
     assert rk.status(ws)[0] == 0
     torch.cuda.synchronize()
     assert torch.equal(i[:k].cpu(), torch.arange(k, dtype=torch.int32))  # all ties: positions
+
+
+@pytest.mark.parametrize("dt", ["f32", "bf16"])
+@pytest.mark.parametrize("largest", [True, False])
+def test_bracket_missed_by_the_sample_falls_back(dt, largest):
+    """Rows whose sampled runs are unrepresentative: values planted on exactly
+    the positions the sample reads put the k-th key outside the sampled
+    bracket (above it in one row, below it in another), beside a row the
+    bracket settles.  The missed rows take the digit passes and every row
+    still equals the oracle."""
+    import radik_amd as rk
+    import radik_amd.workloads as W
+    n = 1 << 18
+    esz = 4 if dt == "f32" else 2
+    pack = 16 // esz
+    lens = np.full(3, n, np.int64)
+    off = np.zeros(4, np.int64)
+    off[1:] = np.cumsum(lens)
+    ks = np.array([n // 2, n // 4, n // 3], np.int32)
+    bits = _bits(dt, rk.GEN_NORMAL, int(off[-1]), 57)
+    big = 100.0 if largest else -100.0
+    conv = (lambda v: np.float32(v).view(np.uint32)) if dt == "f32" else \
+        (lambda v: np.uint16(np.float32(v).view(np.uint32) >> 16))
+    for r, v in ((0, big), (1, -big)):
+        a0 = ((-int(off[r]) * esz) % 16) // esz  # the row's first 16-byte aligned element
+        pos = W.sample_positions(n, 1024, 16, pack, a0, full=True)
+        bits[int(off[r]) + pos] = conv(v)
+    st = _check(dt, bits, off, ks, LARGEST if largest else 0)
+    assert st[2] == 3  # every row took the ordered emit
