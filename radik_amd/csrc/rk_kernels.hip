@@ -633,9 +633,13 @@ __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
       const uint32_t t_lo = block_radix_select_from<SEL_NT, 11, uint32_t, false>(
           keys, s, r_lo, kor ^ kand, kand, hist, scratch, outv);
       const uint32_t ge_lo = outv[3];
-      // sampled keys in [T_lo, T_hi] (at least those of equal keys), scaled
+      // sampled keys in [T_lo, T_hi] (at least those of equal keys), scaled.
+      // The bracket pays while its finisher's one-workgroup select stays
+      // short: up to about 64K keys inside (measured on MI355X: 2^22-key
+      // rows with k = N/4 and N/2, ~130K keys inside, were 2-5 % slower
+      // than the digit passes; k = N/100 there, ~30K inside, 1.6x faster).
       const double est = (double)(ge_lo - ge_hi + 1u) * (double)len / (double)s;
-      const bool fits = est * 1.25 <= (double)st0.cap;
+      const bool fits = est * 1.25 <= (double)st0.cap && est <= 65536.0;
       exact_start_row<SEL_NT>(p, r, fits ? FB_NONE : FB_SAMPLE_TOO_MANY,
                               fits ? (uint32_t)PH_BRACKET : (uint32_t)PH_HIST);
       if (threadIdx.x == 0) {
