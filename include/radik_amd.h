@@ -55,8 +55,9 @@ enum {
 
 /* Largest k for sorted output on the current device (16384 on MI355X: the
  * LDS sort).  Unsorted calls accept any k: rows whose k exceeds this limit
- * (e.g. a median, k = len / 2) go through the exact digit passes and an
- * ordered emit in position order, with no candidate buffer. */
+ * (e.g. a median, k = len / 2) are written by an ordered emit in position
+ * order, after a sampled bracket of the k-th key (two reads of the row) or,
+ * on the exact path and for rows the bracket misses, the digit passes. */
 int32_t rk_max_k(void);
 
 /* Workspace bytes for a batched call over at most `num_rows` rows with at
@@ -152,9 +153,10 @@ int rk_topk_compat_i64(int dtype, const void* valIn, const int64_t* idxIn,
                        int32_t K, uint32_t flags, void* stream);
 
 /* Waits for `stream`, then reads the device status of the last call that
- * used `workspace`: 0 when every row was valid.  Optional outputs: rows that
- * left the sampled path for exact passes, and rows that needed the ordered
- * emit.  No counterpart in the reference (it throws, error_check.hpp:14-58). */
+ * used `workspace`: 0 when every row was valid.  Optional outputs: rows
+ * finished by the exact-pass executor (rows that left the sampled path, and
+ * rows with k above rk_max_k()), and rows that needed the ordered emit.  No
+ * counterpart in the reference (it throws, error_check.hpp:14-58). */
 int rk_topk_status(const void* workspace, void* stream, uint32_t* status,
                    uint32_t* fallback_rows, uint32_t* ordered_rows);
 
@@ -162,11 +164,13 @@ int rk_topk_status(const void* workspace, void* stream, uint32_t* status,
 const char* rk_strerror(int code);
 
 /* Tunables of the batched path: tile size (keys per streaming work unit,
- * default 16384), persistent grid of the streaming kernels (default 8
- * workgroups per CU), sample runs and keys per run (1024 x 16), digit width
- * of the exact passes (8..11 bits, default 11: 3 passes over 32-bit keys)
- * and 16-byte loads in flight per thread (2, 4 or 8; default 4).  A field of
- * 0 selects its default. */
+ * default 16384, smaller for small calls), persistent grid of the streaming
+ * kernels (default 4 workgroups per CU for the sampled path's filter; 1, or
+ * 2 from 2^26 keys, for an RK_EXACT call's executor), sample runs and keys
+ * per run (1024 x 16), digit width of the exact passes (8..11 bits, default
+ * 11: 3 passes over 32-bit keys) and 16-byte loads in flight per thread (2,
+ * 4 or 8; default 2, and 4 for RK_EXACT calls).  A field of 0 selects its
+ * default. */
 typedef struct rk_tune {
   int32_t tile;
   int32_t grid;
