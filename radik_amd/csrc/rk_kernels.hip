@@ -342,8 +342,10 @@ __global__ __launch_bounds__(256) void k_plan_rows(KParams p) {
 // workgroup; rows are processed in chunks of 1024 with a running carry).
 // FUSED (at most 1024 rows): the same workgroup plans the rows first, which
 // saves a launch on small batches.
-template <int DT, bool FUSED>
-__global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
+// NT threads (1024, or one wave for calls of at most 64 rows, where the
+// workgroup barriers of 16 waves would be most of its latency).
+template <int DT, bool FUSED, int NT = 1024>
+__global__ __launch_bounds__(NT) void k_plan_scan(KParams p) {
   // FUSED: thread t plans row t and keeps its plan in registers; otherwise
   // the plans are read back from what k_plan_rows wrote.
   RowPlan mine{0, 0, 0u, 0u, (uint32_t)PH_DONE, false};
@@ -368,7 +370,7 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
   __syncthreads();
   uint32_t carry_t = 0;
   uint64_t carry_c = 0;
-  for (int base = 0; base < p.R; base += 1024) {
+  for (int base = 0; base < p.R; base += NT) {
     const int r = base + tid;
     RowPlan rp{0, 0, 0u, 0u, (uint32_t)PH_DONE, false};
     if (r < p.R) rp = plan_of(r);
@@ -377,14 +379,14 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
     if (rp.bad) atomicOr(&bad_sh, 1u);
     if (rp.phase == PH_HIST) atomicAdd(&hist_sh, 1u);
     uint32_t tot_t;
-    const uint32_t ex_t = block_excl_scan<1024>(t, scratch, tot_t);
+    const uint32_t ex_t = block_excl_scan<NT>(t, scratch, tot_t);
     // 64-bit scan of the capacities: wave scan + per-wave totals.
     const uint64_t inc = wave_incl_scan64(c);
     const int lane = lane_id(), wid = tid >> 6;
     if (lane == 63) scratch64[wid] = inc;
     __syncthreads();
     uint64_t pre = 0, tot_c = 0;
-    for (int w = 0; w < 16; ++w) {
+    for (int w = 0; w < NT / 64; ++w) {
       const uint64_t v = scratch64[w];
       pre += (w < wid) ? v : 0ull;
       tot_c += v;
@@ -440,7 +442,7 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
       kc = (uint32_t)min(31 - __builtin_clz(k), 15);
       lc = (uint32_t)min(63 - __builtin_clzll(len), 31);
     };
-    for (int r = tid; r < p.R; r += 1024) {
+    for (int r = tid; r < p.R; r += NT) {
       uint32_t kc, lc;
       classes(r, kc, lc);
       atomicAdd(&bucket[0][kc], 1u);
@@ -456,7 +458,7 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
       }
     }
     __syncthreads();
-    for (int r = tid; r < p.R; r += 1024) {
+    for (int r = tid; r < p.R; r += NT) {
       uint32_t kc, lc;
       classes(r, kc, lc);
       p.order[atomicAdd(&bucket[0][kc], 1u)] = (uint32_t)r;
@@ -465,7 +467,7 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
   }
   // On a capacity overflow no long row is processed (the status reports it).
   if (status_sh & (ST_TILES | ST_CANDIDATES)) {
-    for (int r = tid; r < p.R; r += 1024) {
+    for (int r = tid; r < p.R; r += NT) {
       const uint32_t ph = FUSED ? mine.phase : p.st[r].phase;
       if (ph == PH_SAMPLE || ph == PH_HIST) p.st[r].phase = PH_DONE;
       p.tile_start[r] = 0;
@@ -475,9 +477,9 @@ __global__ __launch_bounds__(1024) void k_plan_scan(KParams p) {
   // Chunk -> row table of the filter: cleared, then filled for the rows that
   // skip sampling (the sample kernel fills it for the sampled rows).
   const ChunkGrid g(carry_t, (uint32_t)p.chunk_tiles);  // carry_t: total tiles (uniform)
-  for (uint32_t c = tid; c < g.nch; c += 1024) p.chunk_row[c] = NO_ROW;
+  for (uint32_t c = tid; c < g.nch; c += NT) p.chunk_row[c] = NO_ROW;
   __syncthreads();
-  for (int r = tid; r < p.R; r += 1024) {
+  for (int r = tid; r < p.R; r += NT) {
     if ((FUSED ? mine.phase : p.st[r].phase) != PH_FILTER) continue;
     const uint32_t t0 = FUSED ? my_t0 : p.tile_start[r];
     const uint32_t t1 = FUSED ? my_t0 + mine.tiles : p.tile_start[r + 1];
@@ -2533,7 +2535,8 @@ static hipError_t run_typed_u(const KParams& p, const Launch& L0) {
   }
   if (L0.part != PART_FALLBACK) {
     if (p.R <= 1024) {
-      k_plan_scan<DT, true><<<1, 1024, 0, s>>>(p);
+      if (p.R <= 64) k_plan_scan<DT, true, 64><<<1, 64, 0, s>>>(p);
+      else k_plan_scan<DT, true><<<1, 1024, 0, s>>>(p);
     } else {
       k_plan_rows<DT><<<rows_blocks, 256, 0, s>>>(p);
       k_plan_scan<DT, false><<<1, 1024, 0, s>>>(p);
