@@ -179,6 +179,21 @@ RK_DEV float bload4f(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
 // return zeros).
 constexpr uint32_t RSRC_OOB = 0xfffffff0u;
 
+// 16-byte buffer load straight into LDS (gfx950): the wave's active lanes
+// write the 16-byte slots dst + 16 * lane (dst wave-uniform); offsets
+// outside the descriptor's range write zeros.  No VGPR is written; the load
+// counts in vmcnt until its data is in LDS, and the compiler waits for it
+// (s_waitcnt vmcnt) before LDS accesses it cannot tell apart from its slots.
+RK_DEV void bload16_lds(__amdgpu_buffer_rsrc_t r, uint32_t byte_off, unsigned char* dst) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)dst, 16,
+                                           (int)byte_off, 0, 0, 0);
+}
+// The same with 4 bytes per lane (slots dst + 4 * lane).
+RK_DEV void bload4_lds(__amdgpu_buffer_rsrc_t r, uint32_t byte_off, unsigned char* dst) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)dst, 4,
+                                           (int)byte_off, 0, 0, 0);
+}
+
 // Unpacked 16-byte vector of PACK elements (bit patterns, low element first).
 template <int DT>
 struct Vec {
