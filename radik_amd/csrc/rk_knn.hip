@@ -521,26 +521,44 @@ __global__ __launch_bounds__(KNN_NT) __attribute__((amdgpu_waves_per_eu(MODE == 
 template <int DT, int KS>
 __global__ __launch_bounds__(256) void k_knn_xnorm(const void* x, int64_t n, int32_t d, float* out) {
   using U = typename Traits<DT>::U;
-  // two threads per vector, one per half; all of a half's loads in flight
-  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t j = g >> 1;
-  const int h = (int)(g & 1);
-  const bool ok = j < n;
-  const U* row = static_cast<const U*>(x) + (ok ? j : 0) * d + h * 8;
-  uint4 c[KS];
+  // 128 vectors per workgroup.  The block's 16-byte chunks are read with
+  // consecutive lanes on consecutive chunks (whole cache lines per load)
+  // into LDS rows padded by two chunks, so that the reads below, thread
+  // (vector v, half h) taking chunk 2t + h of row v, fall in 16 distinct
+  // bank groups.  Each thread then sums its half in the same order as before.
+  // Measured on MI355X (2^20 x 128 bf16): 54.7 -> 44.0 us against two
+  // threads per vector loading their half's chunks straight from memory.
+  constexpr int VPB = 128, CPV = 2 * KS, STR = CPV + 2, PER = VPB * CPV / 256;
+  __shared__ uint4 rows[VPB * STR];
+  const int tid = threadIdx.x;
+  const int64_t v0 = (int64_t)blockIdx.x * VPB;
+  const U* X = static_cast<const U*>(x);
+  const int nch = d / 8;  // 16-byte chunks per vector (d is a multiple of 16)
+  uint4 c[PER];
 #pragma unroll
-  for (int t = 0; t < KS; ++t)
-    c[t] = (ok && t < d / 16) ? ldg16<false>(row + t * 16) : make_uint4(0u, 0u, 0u, 0u);
+  for (int u = 0; u < PER; ++u) {
+    const int ci = tid + u * 256, v = ci / CPV, k = ci % CPV;
+    const int64_t j = v0 + v;
+    c[u] = (j < n && k < nch) ? ldg16<false>(X + j * d + k * 8) : make_uint4(0u, 0u, 0u, 0u);
+  }
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int ci = tid + u * 256;
+    rows[(ci / CPV) * STR + ci % CPV] = c[u];
+  }
+  __syncthreads();
+  const int v = tid >> 1, h = tid & 1;
   float acc = 0.0f;
 #pragma unroll
-  for (int t = 0; t < KS; ++t) acc = sq8<DT>(c[t], acc);
+  for (int t = 0; t < KS; ++t) acc = sq8<DT>(rows[v * STR + 2 * t + h], acc);
   const float other = __shfl_xor(acc, 1, 64);
-  if (ok && h == 0) out[j] = __fadd_rn(acc, other);
+  const int64_t j = v0 + v;
+  if (j < n && h == 0) out[j] = __fadd_rn(acc, other);
 }
 
 template <int DT>
 void launch_xnorm(const void* x, int64_t n, int32_t d, float* out, hipStream_t s) {
-  const unsigned nb = (unsigned)((2 * n + 255) / 256);
+  const unsigned nb = (unsigned)((n + 127) / 128);
   if (d <= 32) k_knn_xnorm<DT, 2><<<nb, 256, 0, s>>>(x, n, d, out);
   else if (d <= 64) k_knn_xnorm<DT, 4><<<nb, 256, 0, s>>>(x, n, d, out);
   else if (d <= 128) k_knn_xnorm<DT, 8><<<nb, 256, 0, s>>>(x, n, d, out);
