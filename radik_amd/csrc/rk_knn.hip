@@ -696,8 +696,9 @@ __global__ __launch_bounds__(KNN_SEL_NT) void k_knn_select(int pass, const uint3
                                                            float* out_v, int32_t* out_i) {
   extern __shared__ uint64_t sbuf[];
   __shared__ uint32_t hist[HIST_MAX_BINS];
+  __shared__ uint32_t ctab[256];
   __shared__ uint32_t scratch[16];
-  __shared__ uint64_t scratch64[16];
+  __shared__ uint64_t scratch64[3 * KNN_SEL_NT / 64];
   __shared__ uint32_t outv[4];
   __shared__ uint32_t slot_cnt;
   const int32_t nl = nlist_dev ? (int32_t)*nlist_dev : nq;
@@ -729,44 +730,64 @@ __global__ __launch_bounds__(KNN_SEL_NT) void k_knn_select(int pass, const uint3
     const bool in_lds = n <= (uint32_t)lds_keys;
     if (in_lds) load_to_lds<KNN_SEL_NT, 8>(sbuf, src, n, [](uint64_t x, uint32_t) { return x; });
     __syncthreads();
-    const uint64_t kth =
-        in_lds ? block_radix_select<KNN_SEL_NT, 11, uint64_t, false>(sbuf, n, k, hist, scratch, scratch64, outv)
-               : block_radix_select<KNN_SEL_NT, 11, uint64_t, true>(src, n, k, hist, scratch, scratch64, outv);
-    if (threadIdx.x == 0) slot_cnt = 0;
-    __syncthreads();
-    // Keep the k keys >= kth (composite keys are distinct), re-keyed so that
-    // one descending sort gives the output order: best first, or database
-    // order ((~index << 32) | key).
-    constexpr int CH = 4;
-    for (uint32_t c0 = 0; c0 < n; c0 += KNN_SEL_NT * CH) {
-      uint64_t reg[CH];
-#pragma unroll
-      for (int j = 0; j < CH; ++j) {
-        const uint32_t i = c0 + threadIdx.x + j * KNN_SEL_NT;
-        reg[j] = i < n ? (in_lds ? sbuf[i] : src[i]) : 0ull;
+    // Up to half the LDS buffer of candidates: one bucket sort of all of them
+    // (the other half groups them; composite keys are distinct) leaves the
+    // selection best first in sbuf[0, k), so the radix select and the
+    // compaction are skipped (measured on MI355X, 1024 queries with ~600
+    // candidates, k = 100: 28.2 -> 18.5 us).  Keys that do not spread over the
+    // buckets are left untouched, and the radix select runs.
+    const bool whole = in_lds && n > 64 && n <= (uint32_t)lds_keys / 2 &&
+                       bucket_sort_desc<KNN_SEL_NT>(sbuf, sbuf + lds_keys / 2, n, hist, ctab,
+                                                    scratch64, scratch);
+    if (whole) {
+      if (!sorted) {  // database order: the k selected keys re-keyed, sorted below
+        for (uint32_t i = threadIdx.x; i < (uint32_t)k; i += KNN_SEL_NT) {
+          const uint64_t x = sbuf[i];
+          sbuf[i] = ((uint64_t)(uint32_t)x << 32) | (x >> 32);
+        }
       }
+    } else {
+      const uint64_t kth =
+          in_lds ? block_radix_select<KNN_SEL_NT, 11, uint64_t, false>(sbuf, n, k, hist, scratch, scratch64, outv)
+                 : block_radix_select<KNN_SEL_NT, 11, uint64_t, true>(src, n, k, hist, scratch, scratch64, outv);
+      if (threadIdx.x == 0) slot_cnt = 0;
       __syncthreads();
+      // Keep the k keys >= kth (composite keys are distinct), re-keyed so that
+      // one descending sort gives the output order: best first, or database
+      // order ((~index << 32) | key).
+      constexpr int CH = 4;
+      for (uint32_t c0 = 0; c0 < n; c0 += KNN_SEL_NT * CH) {
+        uint64_t reg[CH];
 #pragma unroll
-      for (int j = 0; j < CH; ++j) {
-        const uint32_t i = c0 + threadIdx.x + j * KNN_SEL_NT;
-        const bool take = i < n && reg[j] >= kth;
-        const uint64_t m = __ballot(take);
-        if (m) {
-          const int fl = __ffsll((unsigned long long)m) - 1;
-          uint32_t base = 0;
-          if (lane_id() == fl) base = atomicAdd(&slot_cnt, (uint32_t)__popcll(m));
-          base = (uint32_t)__builtin_amdgcn_readlane((int)base, fl) + mbcnt(m);
-          if (take && base < (uint32_t)k)
-            sbuf[base] = sorted ? reg[j]
-                                : (((uint64_t)(uint32_t)reg[j] << 32) | (reg[j] >> 32));
+        for (int j = 0; j < CH; ++j) {
+          const uint32_t i = c0 + threadIdx.x + j * KNN_SEL_NT;
+          reg[j] = i < n ? (in_lds ? sbuf[i] : src[i]) : 0ull;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+          const uint32_t i = c0 + threadIdx.x + j * KNN_SEL_NT;
+          const bool take = i < n && reg[j] >= kth;
+          const uint64_t m = __ballot(take);
+          if (m) {
+            const int fl = __ffsll((unsigned long long)m) - 1;
+            uint32_t base = 0;
+            if (lane_id() == fl) base = atomicAdd(&slot_cnt, (uint32_t)__popcll(m));
+            base = (uint32_t)__builtin_amdgcn_readlane((int)base, fl) + mbcnt(m);
+            if (take && base < (uint32_t)k)
+              sbuf[base] = sorted ? reg[j]
+                                  : (((uint64_t)(uint32_t)reg[j] << 32) | (reg[j] >> 32));
+          }
         }
       }
     }
     __syncthreads();
-    const uint32_t P2 = next_pow2((uint32_t)k);
-    for (uint32_t i = (uint32_t)k + threadIdx.x; i < P2; i += KNN_SEL_NT) sbuf[i] = 0;
-    __syncthreads();
-    if (P2 > 1) block_bitonic_desc<KNN_SEL_NT>(sbuf, P2);
+    if (!(whole && sorted)) {
+      const uint32_t P2 = next_pow2((uint32_t)k);
+      for (uint32_t i = (uint32_t)k + threadIdx.x; i < P2; i += KNN_SEL_NT) sbuf[i] = 0;
+      __syncthreads();
+      if (P2 > 1) block_bitonic_desc<KNN_SEL_NT>(sbuf, P2);
+    }
     for (uint32_t i = threadIdx.x; i < (uint32_t)k; i += KNN_SEL_NT) {
       const uint64_t v = sbuf[i];
       const uint32_t sk = sorted ? (uint32_t)(v >> 32) : (uint32_t)v;
