@@ -6,6 +6,8 @@
 // ordered, has no host round trip and can be captured in a hipGraph:
 //
 //   plan_rows, plan_scan            row descriptors, tile list, candidate slots
+//                                   (one-row sampled calls: done by the sample
+//                                   kernel's one workgroup instead)
 //   sample                          long rows: threshold T from a sample
 //   filter                          one streaming read: compact keys >= T
 //   select_finalize                 per row: sort (or radix select, then
@@ -344,8 +346,8 @@ __global__ __launch_bounds__(256) void k_plan_rows(KParams p) {
 // saves a launch on small batches.
 // NT threads (1024, or one wave for calls of at most 64 rows, where the
 // workgroup barriers of 16 waves would be most of its latency).
-template <int DT, bool FUSED, int NT = 1024>
-__global__ __launch_bounds__(NT) void k_plan_scan(KParams p) {
+template <int DT, bool FUSED, int NT>
+RK_DEV void plan_scan(const KParams& p) {
   // FUSED: thread t plans row t and keeps its plan in registers; otherwise
   // the plans are read back from what k_plan_rows wrote.
   RowPlan mine{0, 0, 0u, 0u, (uint32_t)PH_DONE, false};
@@ -488,6 +490,11 @@ __global__ __launch_bounds__(NT) void k_plan_scan(KParams p) {
   }
 }
 
+template <int DT, bool FUSED, int NT = 1024>
+__global__ __launch_bounds__(NT) void k_plan_scan(KParams p) {
+  plan_scan<DT, FUSED, NT>(p);
+}
+
 // Sends row r to the exact passes from its first digit: histogram cleared,
 // nothing resolved, all of its k still to take, and the row added to the
 // exact-pass executor's list.  Called by a whole workgroup of NT threads of
@@ -536,7 +543,10 @@ RK_DEV void exact_start_row(const KParams& p, int r, uint32_t why, uint32_t phas
 // k then holds unless the sample is unrepresentative, which the select kernel
 // detects exactly (it counts) and sends to the exact passes.
 // ---------------------------------------------------------------------------
-template <int DT, bool LARGEST>
+// PLAN (calls of one row, launched as one workgroup): the workgroup plans
+// the call first (plan_scan), which saves the plan's launch; its barriers
+// make the plan visible to the workgroup, and later launches see it anyway.
+template <int DT, bool LARGEST, bool PLAN = false>
 __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
   using U = typename Traits<DT>::U;
   extern __shared__ uint64_t dyn64[];
@@ -546,6 +556,10 @@ __global__ __launch_bounds__(SEL_NT) void k_sample(KParams p) {
   __shared__ uint64_t scratch64[16];
   __shared__ uint32_t outv[4];
   __shared__ uint32_t next_sh;
+  if constexpr (PLAN) {
+    plan_scan<DT, true, SEL_NT>(p);
+    __syncthreads();
+  }
   for (;;) {
     if (threadIdx.x == 0) next_sh = atomicAdd(&p.hdr->q_sample, 1u);
     __syncthreads();
@@ -2409,6 +2423,8 @@ template <int DT, bool LARGEST>
 static void set_lds_attrs(size_t dyn) {
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sample<DT, LARGEST>),
                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sample<DT, LARGEST, true>),
+                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_select_finalize<DT, LARGEST>),
                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_select_finalize<DT, LARGEST, true>),
@@ -2533,7 +2549,10 @@ static hipError_t run_typed_u(const KParams& p, const Launch& L0) {
     const int cus = device_cus();
     fb_sel = std::min(fb_sel, cus);
   }
-  if (L0.part != PART_FALLBACK) {
+  // A sampled call of one row: its sample workgroup plans the call first
+  // (one launch fewer on the latency-bound single-row path).
+  const bool plan_in_sample = !exact_call && p.R == 1 && L0.part != PART_FALLBACK;
+  if (L0.part != PART_FALLBACK && !plan_in_sample) {
     if (p.R <= 1024) {
       if (p.R <= 64) k_plan_scan<DT, true, 64><<<1, 64, 0, s>>>(p);
       else k_plan_scan<DT, true><<<1, 1024, 0, s>>>(p);
@@ -2543,7 +2562,8 @@ static hipError_t run_typed_u(const KParams& p, const Launch& L0) {
     }
   }
   if (!(p.flags & FL_EXACT) && L0.part != PART_FALLBACK) {
-    k_sample<DT, LARGEST><<<g_sample, SEL_NT, dyn_sample, s>>>(p);
+    if (plan_in_sample) k_sample<DT, LARGEST, true><<<1, SEL_NT, dyn_sample, s>>>(p);
+    else k_sample<DT, LARGEST><<<g_sample, SEL_NT, dyn_sample, s>>>(p);
     std::pair<hipEvent_t, hipEvent_t> ev;
     const bool timed = timing_begin(s, ev);
     k_filter<DT, LARGEST, U><<<fgrid, STREAM_NT, 0, s>>>(p);
